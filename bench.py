@@ -1,0 +1,211 @@
+"""Benchmark: bootstrapped NAND gates/s on one MI355X, split-FP32 vs FP64 external product.
+
+Workload (one "step"): 16384 independent NAND gates, each a full gate bootstrap (pre-combination,
+modulus switch, n = 630 MuxRotate steps, extraction) plus keyswitch, on ciphertexts already
+resident in HBM.  Parameters N=1024, k=1, n=630, l=3, Bgbit=7, keyswitch t=8, basebit=2; keys from
+seed 0; uniformly random input bits (numpy PCG64, seed 1).
+
+Prints one JSON line.  `value` is the split-FP32 rate; the FP64 rate, their ratio, the chained
+10^6-gate decryption audit, the roofline of the bootstrap kernel (timed with HIP events on the
+library stream) and the CPU baseline (the oracle's FP64 FFT path on the host cores) ride along.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fpga-tfhe_amd"))
+
+METRIC = "bootstrapped NAND gates/sec on 1 MI355X, split-FP32 vs FP64 at 0 fails/1e6"
+N, n, k, l, Bgbit = 1024, 630, 1, 3, 7
+M = N // 2
+# algorithmic work per gate: n * [8 transforms * (5 M log2 M + 6 M) + (k+1) l (k+1) M complex MACs * 8]
+FLOP_PER_GATE = n * (8 * (5 * M * 9 + 6 * M) + (k + 1) * l * (k + 1) * M * 8)   # 162,570,240
+PEAK_TFLOPS = {"fp64": 78.6, "split32": 157.3}  # MI355X published vector/matrix peaks
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--audit-gates", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prec", choices=["both", "split32", "fp64"], default="both")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_path(T, dk, prec, bufs, batch, steps, warmup):
+    d_out, d_a, d_b = bufs
+    name = "bootstrap_fp64" if prec == T.PREC_FP64 else "bootstrap_split32"
+    for _ in range(warmup):
+        dk.gate_dev("NAND", d_out.ptr, d_a.ptr, d_b.ptr, None, batch, prec=prec)
+    T.sync()
+    T.profile_reset()
+    T.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dk.gate_dev("NAND", d_out.ptr, d_a.ptr, d_b.ptr, None, batch, prec=prec)
+    T.sync()
+    t1 = time.perf_counter()
+    T.profile_enable(False)
+    boot_ms, boot_n = T.profile_read(name)
+    ks_ms, ks_n = T.profile_read("keyswitch")
+    return {"ms_per_step": (t1 - t0) * 1e3 / steps, "gates_per_s": batch * steps / (t1 - t0),
+            "boot_ms_avg": boot_ms / max(boot_n, 1), "ks_ms_avg": ks_ms / max(ks_n, 1)}
+
+
+def audit(T, dk, ks, prec, batch, total, rng):
+    """Chained NAND levels: each level's outputs are the next level's inputs (random pairing), so
+    every gate sees bootstrapped-and-keyswitched noise.  Decryption on the GPU, plaintext on host."""
+    bits = rng.integers(0, 2, batch).astype(np.int32)
+    cur = T.DeviceBuffer.from_array(ks.encrypt(bits))
+    nxt_a = T.DeviceBuffer(cur.nbytes)
+    nxt_b = T.DeviceBuffer(cur.nbytes)
+    out = T.DeviceBuffer(cur.nbytes)
+    done = fails = 0
+    try:
+        while done < total:
+            host = cur.to_array((batch, n + 1))
+            perm = rng.permutation(batch)
+            T.lib().tfhe_gpu_memcpy_h2d(nxt_b.ptr, np.ascontiguousarray(host[perm]).ctypes.data_as(T._vp),
+                                        host.nbytes)
+            dk.gate_dev("NAND", out.ptr, cur.ptr, nxt_b.ptr, None, batch, prec=prec)
+            got = T.decrypt_dev(out, ks.lwe_key, batch)
+            bits = 1 - (bits & bits[perm])
+            bad = got != bits
+            fails += int(bad.sum())
+            bits = got  # follow the decrypted values so one failure is counted once
+            cur, out = out, cur
+            done += batch
+    finally:
+        for b in (cur, nxt_a, nxt_b, out):
+            b.free()
+    return done, fails
+
+
+def cpu_baseline(ks, batch_bits, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    keys = O.Keys(O.PARAMS_1024, ks.lwe_key, ks.tlwe_key, ks.ksk, ks.bk)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    try:
+        cal = 2 * threads
+        ca, cb = ks.encrypt(batch_bits[0][:cal]), ks.encrypt(batch_bits[1][:cal])
+        _, sec = O.gate_batch(keys, "NAND", ca, cb, nthreads=threads)
+        cnt = int(max(cal, min(20000, seconds * cal / max(sec, 1e-6))))
+        cnt = (cnt // threads) * threads or threads
+        ca, cb = ks.encrypt(batch_bits[0][:cnt]), ks.encrypt(batch_bits[1][:cnt])
+        out, sec = O.gate_batch(keys, "NAND", ca, cb, nthreads=threads)
+        ok = bool(np.array_equal(ks.decrypt(out), 1 - (batch_bits[0][:cnt] & batch_bits[1][:cnt])))
+        try:
+            model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")][0]
+        except Exception:
+            model = "unknown"
+        return {"value": cnt / sec, "unit": "gates/s", "cores": threads, "kind": "port",
+                "sample": f"{cnt} NAND gates (bootstrap + keyswitch, FP64 FFT path), {sec:.1f} s, {model}",
+                "correct": ok}
+    finally:
+        keys.free()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import tfhe_amd as T
+    T.init(0)
+    dev = T.device_info()
+    log(f"[bench] device {dev}")
+    t = time.perf_counter()
+    ks = T.KeySet(1024, seed=(0,))
+    dk = T.DeviceKey.from_keyset(ks)
+    log(f"[bench] keys ready in {time.perf_counter() - t:.1f} s")
+    rng = np.random.default_rng(1)
+    B = args.batch
+    bits = rng.integers(0, 2, (2, B)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    d_a, d_b = T.DeviceBuffer.from_array(ca), T.DeviceBuffer.from_array(cb)
+    d_out = T.DeviceBuffer(ca.nbytes)
+    want = 1 - (bits[0] & bits[1])
+
+    precs = {"split32": T.PREC_SPLIT32, "fp64": T.PREC_FP64}
+    if args.prec != "both":
+        precs = {args.prec: precs[args.prec]}
+    res = {}
+    for name, prec in precs.items():
+        r = time_path(T, dk, prec, (d_out, d_a, d_b), B, args.steps, args.warmup)
+        r["correct"] = bool(np.array_equal(T.decrypt_dev(d_out, ks.lwe_key, B), want))
+        res[name] = r
+        log(f"[bench] {name}: {r}")
+
+    head = "split32" if "split32" in res else "fp64"
+    aud = None
+    if args.audit_gates > 0:
+        t = time.perf_counter()
+        done, fails = audit(T, dk, ks, precs[head], B, args.audit_gates, np.random.default_rng(2))
+        aud = {"gates": done, "failures": fails, "precision": head, "chained": True}
+        log(f"[bench] audit {aud} in {time.perf_counter() - t:.1f} s")
+
+    cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        cpu = cpu_baseline(ks, bits, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu} in {time.perf_counter() - t:.1f} s")
+
+    hr = res[head]
+    achieved = FLOP_PER_GATE * B / (hr["boot_ms_avg"] * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_bootstrap_{head}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": round(hr["gates_per_s"], 1),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "split32" else "f64",
+        "data": "synthetic: uniform random bits (PCG64 seed 1) encrypted under keys from seed 0",
+        "config": {"workload": f"{B} independent NAND gate bootstraps + keyswitch per step, "
+                               f"{'split-FP32' if head == 'split32' else 'FP64'} external product",
+                   "N": N, "k": k, "n": n, "l": l, "Bgbit": Bgbit, "ks_t": 8, "ks_basebit": 2, "batch": B},
+        "paths": {kk: {"gates_per_s": round(v["gates_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 3),
+                       "bootstrap_kernel_ms": round(v["boot_ms_avg"], 3), "keyswitch_kernel_ms": round(v["ks_ms_avg"], 3),
+                       "correct": v["correct"]} for kk, v in res.items()},
+        "split32_over_fp64": (round(res["split32"]["gates_per_s"] / res["fp64"]["gates_per_s"], 3)
+                              if len(res) == 2 else None),
+        "audit": aud,
+        "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),
+                     "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
+                     "traffic": traffic, "kernel": f"bootstrap_{head}", "flop_per_gate": FLOP_PER_GATE},
+        "cpu_baseline": cpu,
+        "device": dev,
+    }
+    print(json.dumps(line), flush=True)
+    for b in (d_a, d_b, d_out):
+        b.free()
+    dk.free()
+    ks.free()
+
+
+if __name__ == "__main__":
+    main()

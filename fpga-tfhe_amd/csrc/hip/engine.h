@@ -1,0 +1,80 @@
+// engine.h -- host-side interface of the gfx950 kernels (launchers live in engine.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfhe {
+
+enum Prec { PREC_FP64 = 0, PREC_SPLIT32 = 1 };
+
+// Device parameter set (k = 1 only).
+struct DevParams {
+    int n, N, l, Bgbit, ks_t, ks_basebit;
+};
+
+// Per-transform-size twiddle tables in device memory (built once per N).
+struct DevTwiddles {
+    int N = 0;
+    void* t1_fold64 = nullptr;   // [64][R1] C2<double>: w^{L(1+4k1)}  (twist folded, FP64 path)
+    void* t2_64 = nullptr;       // [64][8]  C2<double>: W_64^{(L&7) k2}
+    void* t1_plain32 = nullptr;  // [64][R1] C2<float>: W_M^{L k1}     (split path)
+    void* t2_32 = nullptr;       // [64][8]  C2<float>
+    void* twist64 = nullptr;     // [64][R1] C2<double>: w^{64a + L}   (split path, FP64 twist)
+    void* wa64 = nullptr;        // [R1]     C2<double>: w^{64a}
+};
+
+// One launch of the fused bootstrap kernel.
+//   mode 0: job g reads x = (0, cst[s]) + alpha[s]*in_a[s][g'] + beta[s]*in_b[s][g'] from segment s
+//           (g' = g - seg_start[s]), modulus-switches it, builds the X^{-barb} test vector of mu,
+//           blind-rotates and writes the extracted sample u_out[g][0..N].
+//   mode 1: acc_io[g] (k+1 = 2 polys) is blind-rotated by bara_in[g][0..n) in place.
+//   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out optional.
+struct BootLaunch {
+    int mode = 0;
+    int count = 0;
+    int nseg = 1;
+    int seg_start[3] = {0, 0, 0};
+    const int32_t* in_a[2] = {nullptr, nullptr};
+    const int32_t* in_b[2] = {nullptr, nullptr};
+    int32_t alpha[2] = {1, 1}, beta[2] = {0, 0}, cst[2] = {0, 0};
+    int32_t mu = 0;
+    int32_t* acc_io = nullptr;
+    const int32_t* bara_in = nullptr;
+    const int32_t* row_set = nullptr;
+    double* raw_out = nullptr;
+    int32_t* u_out = nullptr;
+};
+
+struct DevKeyView {
+    DevParams p;
+    const void* bk64 = nullptr;      // FP64 FFT-domain key, lane order
+    const void* bk32 = nullptr;      // split FP32 key {hi.re, hi.im, lo.re, lo.im}
+    int split_bits = 24;
+    const int32_t* ksk = nullptr;    // [kN][t][base-1][NP]
+    int ksk_np = 0;                  // padded output length (multiple of 64)
+};
+
+hipError_t build_twiddles(DevTwiddles& tw, int N);
+void free_twiddles(DevTwiddles& tw);
+
+bool params_supported(const DevParams& p);
+hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& L,
+                            hipStream_t st);
+// KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.
+hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
+                            int32_t add_b, int count, hipStream_t st);
+// time-domain bk [n][2l][2][N] (device) -> FP64 and/or split FP32 FFT-domain keys
+hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32_t* d_bk, void* bk64, void* bk32,
+                          int split_bits, hipStream_t st);
+// KSK [kN][t][base][n+1] (device) -> [kN][t][base-1][NP]
+hipError_t launch_ksk_relayout(const DevParams& p, const int32_t* d_ksk_in, int32_t* d_ksk_out, int np,
+                               hipStream_t st);
+hipError_t launch_extract(int N, int32_t* u, const int32_t* acc, int count, hipStream_t st);
+hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* digits, const int32_t* acc,
+                                   const int32_t* bara, int count, hipStream_t st);
+hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse, double* out, const void* in,
+                           int count, hipStream_t st);
+hipError_t launch_decrypt(int32_t* bits, const int32_t* samples, const int32_t* key, int n, int count,
+                          hipStream_t st);
+
+}  // namespace tfhe

@@ -1,0 +1,773 @@
+// engine.hip -- gfx950 kernels of the TFHE gate-bootstrapping path and their launchers.
+//
+// Kernels (one 64-lane workgroup = one wavefront per unit of work):
+//   k_boot64 / k_boot32  fused bootstrap: gate pre-combination, modulus switch, test vector,
+//                        n MuxRotate steps with the TLWE accumulator resident in LDS, sample
+//                        extraction.  FP64 transform path / split-FP32 transform path.
+//   k_keyswitch          bit-exact LWE keyswitch, a tile of G ciphertexts per workgroup so each
+//                        keyswitch-key row is read once per tile (coalesced over coefficients).
+//   k_keyprep            time-domain bootstrapping key -> FFT domain (FP64 or FP32 hi/lo).
+//   k_ksk_relayout       keyswitch key -> [kN][t][base-1][NP] without the unused h=0 rows.
+//   k_extract, k_rotdec, k_fft_test, k_decrypt  stage kernels for parity tests and audits.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "tfhe_device.hpp"
+
+namespace tfhe {
+
+constexpr int kMaxLweN = 1024;   // largest LWE dimension the LDS prologue holds
+
+struct BootKArgs {
+    BootLaunch L;
+    const void* bk;
+    const void* t1;
+    const void* t2;
+    const void* twist64;
+    const void* wa64;
+    int n;
+    int split_bits;
+};
+
+template <int N>
+__device__ __forceinline__ constexpr int log2N() {
+    return N == 1024 ? 10 : (N == 2048 ? 11 : (N == 512 ? 9 : 12));
+}
+
+// Gate pre-combination + modulus switch + test vector (mode 0) or accumulator load (modes 1, 2).
+// Modulus switch to Z_2N: ((uint64(x) << 32) + interv/2) / interv with interv = 2^64 / 2N
+// (tfhe_bootstrap_woKS_FFT.cpp:3-10) equals (x + 2^(SH-1)) >> SH on uint32, SH = 32 - log2(2N).
+template <int N>
+__device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lane, int32_t* s_acc,
+                                              int16_t* s_bara) {
+    const int n = A.n;
+    if (A.L.mode == 0) {
+        constexpr int SH = 32 - (log2N<N>() + 1);
+        const int s = (A.L.nseg > 1 && g >= A.L.seg_start[1]) ? 1 : 0;
+        const int gl = g - A.L.seg_start[s];
+        const int32_t* xa = A.L.in_a[s] + (size_t)gl * (n + 1);
+        const int32_t* xb = A.L.in_b[s] ? A.L.in_b[s] + (size_t)gl * (n + 1) : nullptr;
+        const uint32_t al = (uint32_t)A.L.alpha[s], be = (uint32_t)A.L.beta[s];
+        for (int i = lane; i < n; i += 64) {
+            uint32_t x = al * (uint32_t)xa[i];
+            if (xb) x += be * (uint32_t)xb[i];
+            s_bara[i] = (int16_t)((x + (1u << (SH - 1))) >> SH);
+        }
+        uint32_t xbv = (uint32_t)A.L.cst[s] + al * (uint32_t)xa[n];
+        if (xb) xbv += be * (uint32_t)xb[n];
+        const int barb = (int)((xbv + (1u << (SH - 1))) >> SH);
+        // b = X^{2N - barb} [mu, ..., mu] (mulByXaiOrCopy.cpp:310-326), a = 0 (tb :317)
+        const int sft = (2 * N - barb) & (2 * N - 1);
+        for (int j = lane; j < N; j += 64) {
+            s_acc[j] = 0;
+            const int e = (j - sft) & (2 * N - 1);
+            s_acc[N + j] = (e < N) ? A.L.mu : -A.L.mu;
+        }
+    } else {
+        const int32_t* ain = A.L.acc_io + (size_t)g * 2 * N;
+        for (int j = lane; j < 2 * N; j += 64) s_acc[j] = ain[j];
+        if (A.L.mode == 1)
+            for (int i = lane; i < n; i += 64) s_bara[i] = (int16_t)A.L.bara_in[(size_t)g * n + i];
+    }
+}
+
+// Sample extraction at index 0 (upstream tLweExtractLweSample, k = 1) or accumulator write-back.
+template <int N>
+__device__ __forceinline__ void boot_epilogue(const BootKArgs& A, int g, int lane, const int32_t* s_acc) {
+    if (A.L.mode == 0) {
+        int32_t* u = A.L.u_out + (size_t)g * (N + 1);
+        for (int j = lane; j < N; j += 64) u[j] = (j == 0) ? s_acc[0] : (int32_t)(0u - (uint32_t)s_acc[N - j]);
+        if (lane == 0) u[N] = s_acc[N];
+        if (A.L.acc_io) {
+            int32_t* ao = A.L.acc_io + (size_t)g * 2 * N;
+            for (int j = lane; j < 2 * N; j += 64) ao[j] = s_acc[j];
+        }
+    } else {
+        int32_t* ao = A.L.acc_io + (size_t)g * 2 * N;
+        for (int j = lane; j < 2 * N; j += 64) ao[j] = s_acc[j];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 path: one MuxRotate step (tfhe_MuxRotate_FFT.cpp:101-130) or one plain external product.
+//   ROT:  acc <- acc + bk_i [.] ((X^a - 1) acc)
+//   !ROT: acc <- bk_i [.] acc
+// Decomposition and rotation are exact integer work; the external product runs the folded FP64
+// transform with the twist folded into the pass-1 twiddles; the result is rounded to nearest and
+// reduced mod 2^32 (the reference truncates, TorusPolynomial_fft.cpp:40).
+// ---------------------------------------------------------------------------------------------
+template <int N, int L, int BG, bool ROT>
+__device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, int a, const C2<double>* __restrict__ kr,
+                                       const LaneTw<double, N / 128>& tw, const C2<double>* wa, int lane,
+                                       double* raw) {
+    constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    C2<double> o0[R1], o1[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t u[NC];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const int j = lane + 64 * r;
+            u[r] = (ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF;
+        }
+#pragma unroll
+        for (int lvl = 0; lvl < L; ++lvl) {
+            const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M;
+            C2<double> k0[R1], k1[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) { k0[r] = k[r * 64 + lane]; k1[r] = k[M + r * 64 + lane]; }
+            C2<double> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                v[aa] = C2<double>{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
+                if (aa) v[aa] = cmul(v[aa], wa[aa]);
+            }
+            fft_fwd<R1>(v, tw, s_buf, lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) { cmac(o0[r], v[r], k0[r]); cmac(o1[r], v[r], k1[r]); }
+        }
+    }
+    fft_inv<R1>(o0, tw, s_buf, lane);
+    fft_inv<R1>(o1, tw, s_buf, lane);
+    const double sc = 1.0 / (double)M;
+#pragma unroll
+    for (int aa = 0; aa < R1; ++aa) {
+        const int j = lane + 64 * aa;
+        C2<double> z0 = aa ? cmulc(o0[aa], wa[aa]) : o0[aa];
+        C2<double> z1 = aa ? cmulc(o1[aa], wa[aa]) : o1[aa];
+        z0.x *= sc; z0.y *= sc; z1.x *= sc; z1.y *= sc;
+        if (raw) { raw[j] = z0.x; raw[j + M] = z0.y; raw[N + j] = z1.x; raw[N + j + M] = z1.y; }
+        const uint32_t r00 = round_mod32(z0.x), r01 = round_mod32(z0.y);
+        const uint32_t r10 = round_mod32(z1.x), r11 = round_mod32(z1.y);
+        if (ROT) {
+            s_acc[j] = (int32_t)((uint32_t)s_acc[j] + r00);
+            s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + r01);
+            s_acc[N + j] = (int32_t)((uint32_t)s_acc[N + j] + r10);
+            s_acc[N + j + M] = (int32_t)((uint32_t)s_acc[N + j + M] + r11);
+        } else {
+            s_acc[j] = (int32_t)r00; s_acc[j + M] = (int32_t)r01;
+            s_acc[N + j] = (int32_t)r10; s_acc[N + j + M] = (int32_t)r11;
+        }
+    }
+    wave_sync();
+}
+
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64) k_boot64(BootKArgs A) {
+    constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ C2<double> s_buf[M];
+    __shared__ int16_t s_bara[kMaxLweN];
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x;
+    boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    wave_sync();
+
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, (const C2<double>*)A.t1, (const C2<double>*)A.t2, lane);
+    C2<double> wa[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) wa[r] = ((const C2<double>*)A.wa64)[r];
+    const C2<double>* key = (const C2<double>*)A.bk;
+    constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+
+    if (A.L.mode == 2) {
+        double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+        step64<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, wa, lane, raw);
+    } else {
+        const int n = A.n;
+        for (int i = 0; i < n; ++i) {
+            const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+            if (a == 0) continue;  // tfhe_blindRotate_FFT.cpp:181
+            step64<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, wa, lane, nullptr);
+        }
+    }
+    boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Split-FP32 path.  Each key coefficient K = Khi * 2^s + Klo with Klo in [-2^(s-1), 2^(s-1));
+// both halves are stored as FP32 transforms.  Digits are twisted in FP64 and rounded to FP32,
+// transformed and multiplied in FP32, and the hi and lo products are inverse-transformed
+// separately.  Untwist and rounding run in FP64; the hi product is small enough to round to the
+// exact integer, so the 2^s scaling does not amplify FP32 error.
+// ---------------------------------------------------------------------------------------------
+template <int N, int L, int BG, bool ROT>
+__device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, const float4* __restrict__ kr,
+                                       const LaneTw<float, N / 128>& tw, const C2<double>* tws, int lane,
+                                       int split_bits, double* raw) {
+    constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        h0[r] = C2<float>{0.f, 0.f}; l0[r] = C2<float>{0.f, 0.f};
+        h1[r] = C2<float>{0.f, 0.f}; l1[r] = C2<float>{0.f, 0.f};
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t u[NC];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const int j = lane + 64 * r;
+            u[r] = (ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF;
+        }
+#pragma unroll
+        for (int lvl = 0; lvl < L; ++lvl) {
+            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
+            float4 k0[R1], k1[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) { k0[r] = k[r * 64 + lane]; k1[r] = k[M + r * 64 + lane]; }
+            C2<float> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                C2<double> z{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
+                z = cmul(z, tws[aa]);
+                v[aa] = C2<float>{(float)z.x, (float)z.y};
+            }
+            fft_fwd<R1>(v, tw, s_buf, lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
+                cmac(l0[r], v[r], C2<float>{k0[r].z, k0[r].w});
+                cmac(h1[r], v[r], C2<float>{k1[r].x, k1[r].y});
+                cmac(l1[r], v[r], C2<float>{k1[r].z, k1[r].w});
+            }
+        }
+    }
+    fft_inv<R1>(h0, tw, s_buf, lane);
+    fft_inv<R1>(l0, tw, s_buf, lane);
+    fft_inv<R1>(h1, tw, s_buf, lane);
+    fft_inv<R1>(l1, tw, s_buf, lane);
+    const double sc = 1.0 / (double)M;
+    const double hs = ldexp(1.0, split_bits);
+#pragma unroll
+    for (int aa = 0; aa < R1; ++aa) {
+        const int j = lane + 64 * aa;
+        C2<double> zh0 = cmulc(C2<double>{(double)h0[aa].x, (double)h0[aa].y}, tws[aa]);
+        C2<double> zl0 = cmulc(C2<double>{(double)l0[aa].x, (double)l0[aa].y}, tws[aa]);
+        C2<double> zh1 = cmulc(C2<double>{(double)h1[aa].x, (double)h1[aa].y}, tws[aa]);
+        C2<double> zl1 = cmulc(C2<double>{(double)l1[aa].x, (double)l1[aa].y}, tws[aa]);
+        zh0.x *= sc; zh0.y *= sc; zl0.x *= sc; zl0.y *= sc;
+        zh1.x *= sc; zh1.y *= sc; zl1.x *= sc; zl1.y *= sc;
+        const uint32_t r00 = (round_mod32(zh0.x) << split_bits) + round_mod32(zl0.x);
+        const uint32_t r01 = (round_mod32(zh0.y) << split_bits) + round_mod32(zl0.y);
+        const uint32_t r10 = (round_mod32(zh1.x) << split_bits) + round_mod32(zl1.x);
+        const uint32_t r11 = (round_mod32(zh1.y) << split_bits) + round_mod32(zl1.y);
+        if (raw) {
+            raw[j] = zh0.x * hs + zl0.x; raw[j + M] = zh0.y * hs + zl0.y;
+            raw[N + j] = zh1.x * hs + zl1.x; raw[N + j + M] = zh1.y * hs + zl1.y;
+        }
+        if (ROT) {
+            s_acc[j] = (int32_t)((uint32_t)s_acc[j] + r00);
+            s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + r01);
+            s_acc[N + j] = (int32_t)((uint32_t)s_acc[N + j] + r10);
+            s_acc[N + j + M] = (int32_t)((uint32_t)s_acc[N + j + M] + r11);
+        } else {
+            s_acc[j] = (int32_t)r00; s_acc[j + M] = (int32_t)r01;
+            s_acc[N + j] = (int32_t)r10; s_acc[N + j + M] = (int32_t)r11;
+        }
+    }
+    wave_sync();
+}
+
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64) k_boot32(BootKArgs A) {
+    constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ C2<float> s_buf[M];
+    __shared__ int16_t s_bara[kMaxLweN];
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x;
+    boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    wave_sync();
+
+    LaneTw<float, R1> tw;
+    load_lane_tw(tw, (const C2<float>*)A.t1, (const C2<float>*)A.t2, lane);
+    C2<double> tws[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<double>*)A.twist64)[lane * R1 + r];
+    const float4* key = (const float4*)A.bk;
+    constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+    const int sb = A.split_bits;
+
+    if (A.L.mode == 2) {
+        double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+        step32<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, tws, lane, sb, raw);
+    } else {
+        const int n = A.n;
+        for (int i = 0; i < n; ++i) {
+            const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+            if (a == 0) continue;
+            step32<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, tws, lane, sb, nullptr);
+        }
+    }
+    boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Keyswitch (lwe-keyswitch-functions.cpp:114-129, :228-241), bit-exact.
+// Grid (NP/64, ceil(count/G)); lane = output coefficient.  Per input index i the lane loads the
+// t*3 candidate rows ks[i][j][1..3][coef] once and applies them to G ciphertexts; the digit of
+// (ciphertext, i, j) is wave-uniform (scalar), read with v_readlane from a coalesced vector load.
+// ---------------------------------------------------------------------------------------------
+template <int TKS, int G>
+__global__ void __launch_bounds__(64) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
+                                                  int32_t add_b, const int32_t* __restrict__ ksk,
+                                                  int32_t* __restrict__ out, int count, int kN, int n, int np) {
+    const int lane = threadIdx.x;
+    const int coef = blockIdx.x * 64 + lane;
+    const int g0 = blockIdx.y * G;
+    const int gcount = min(G, count - g0);
+    const uint32_t prec_off = 1u << (32 - (1 + 2 * TKS));
+    const size_t ust = (size_t)kN + 1;
+    uint32_t acc[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g] = 0;
+    if (coef == n) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (g < gcount) {
+                uint32_t b = (uint32_t)u1[(g0 + g) * ust + kN] + (uint32_t)add_b;
+                if (u2) b += (uint32_t)u2[(g0 + g) * ust + kN];
+                acc[g] = b;
+            }
+    }
+    for (int i0 = 0; i0 < kN; i0 += 64) {
+        uint32_t av[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            uint32_t v = 0;
+            if (g < gcount) {
+                v = (uint32_t)u1[(g0 + g) * ust + i0 + lane];
+                if (u2) v += (uint32_t)u2[(g0 + g) * ust + i0 + lane];
+            }
+            av[g] = v + prec_off;
+        }
+        for (int ii = 0; ii < 64; ++ii) {
+            const int i = i0 + ii;
+            uint32_t kv[TKS][3];
+            const int32_t* kp = ksk + (size_t)i * TKS * 3 * np + coef;
+#pragma unroll
+            for (int j = 0; j < TKS; ++j)
+#pragma unroll
+                for (int h = 0; h < 3; ++h) kv[j][h] = (uint32_t)kp[(j * 3 + h) * np];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g < gcount) {
+                    const uint32_t ai = (uint32_t)__builtin_amdgcn_readlane((int)av[g], ii);
+#pragma unroll
+                    for (int j = 0; j < TKS; ++j) {
+                        const uint32_t d = (ai >> (30 - 2 * j)) & 3u;
+                        if (d == 1) acc[g] -= kv[j][0];
+                        else if (d == 2) acc[g] -= kv[j][1];
+                        else if (d == 3) acc[g] -= kv[j][2];
+                    }
+                }
+            }
+        }
+    }
+    if (coef <= n) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (g < gcount) out[(size_t)(g0 + g) * (n + 1) + coef] = (int32_t)acc[g];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Key preparation: one wavefront per key polynomial.  Both outputs come from the FP64 transform
+// (twist folded into the pass-1 twiddles), so the FP32 split key is the correctly rounded FP64
+// transform of each exact integer half.
+// ---------------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(64) k_keyprep(const int32_t* __restrict__ bk, C2<double>* __restrict__ out64,
+                                                float4* __restrict__ out32, int split_bits,
+                                                const C2<double>* __restrict__ t1, const C2<double>* __restrict__ t2,
+                                                const C2<double>* __restrict__ wa64) {
+    constexpr int M = N / 2, R1 = M / 64;
+    __shared__ C2<double> s_buf[M];
+    const int lane = threadIdx.x;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * N;
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, t1, t2, lane);
+    C2<double> wa[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) wa[r] = wa64[r];
+    C2<double> v[R1];
+    if (out64) {
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            v[aa] = C2<double>{(double)p[lane + 64 * aa], (double)p[lane + 64 * aa + M]};
+            v[aa] = cmul(v[aa], wa[aa]);
+        }
+        fft_fwd<R1>(v, tw, s_buf, lane);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) out64[poly * M + r * 64 + lane] = v[r];
+    }
+    if (out32) {
+        C2<double> h[R1];
+        const int64_t half = (int64_t)1 << (split_bits - 1);
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const int64_t x0 = p[lane + 64 * aa], x1 = p[lane + 64 * aa + M];
+            const int64_t hi0 = (x0 + half) >> split_bits, hi1 = (x1 + half) >> split_bits;
+            const int64_t lo0 = x0 - (hi0 << split_bits), lo1 = x1 - (hi1 << split_bits);
+            h[aa] = cmul(C2<double>{(double)hi0, (double)hi1}, wa[aa]);
+            v[aa] = cmul(C2<double>{(double)lo0, (double)lo1}, wa[aa]);
+        }
+        fft_fwd<R1>(h, tw, s_buf, lane);
+        fft_fwd<R1>(v, tw, s_buf, lane);
+#pragma unroll
+        for (int r = 0; r < R1; ++r)
+            out32[poly * M + r * 64 + lane] = make_float4((float)h[r].x, (float)h[r].y, (float)v[r].x, (float)v[r].y);
+    }
+}
+
+__global__ void k_ksk_relayout(const int32_t* __restrict__ in, int32_t* __restrict__ out, int kN, int t, int n,
+                               int np) {
+    // out[i][j][h-1][c] = in[i][j][h][c] for h in 1..3, c <= n; zero padding for c > n
+    const size_t total = (size_t)kN * t * 3 * np;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(idx % np);
+        const size_t r = idx / np;  // (i*t + j)*3 + (h-1)
+        const int hm1 = (int)(r % 3);
+        const size_t ij = r / 3;
+        out[idx] = (c <= n) ? in[(ij * 4 + (hm1 + 1)) * (size_t)(n + 1) + c] : 0;
+    }
+}
+
+template <int N>
+__global__ void k_extract(int32_t* __restrict__ u, const int32_t* __restrict__ acc, int count) {
+    const int g = blockIdx.x;
+    const int32_t* a = acc + (size_t)g * 2 * N;
+    int32_t* o = u + (size_t)g * (N + 1);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) o[j] = (j == 0) ? a[0] : (int32_t)(0u - (uint32_t)a[N - j]);
+    if (threadIdx.x == 0) o[N] = a[N];
+}
+
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64) k_rotdec(int32_t* __restrict__ rot, int32_t* __restrict__ digits,
+                                               const int32_t* __restrict__ acc, const int32_t* __restrict__ bara) {
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    __shared__ int32_t s_acc[2 * N];
+    const int g = blockIdx.x, lane = threadIdx.x;
+    for (int j = lane; j < 2 * N; j += 64) s_acc[j] = acc[(size_t)g * 2 * N + j];
+    __syncthreads();
+    const int a = bara[g];
+    for (int q = 0; q < 2; ++q)
+        for (int j = lane; j < N; j += 64) {
+            const uint32_t t = rotdiff<N>(s_acc + q * N, j, a);
+            rot[((size_t)g * 2 + q) * N + j] = (int32_t)t;
+            const uint32_t uu = t + OFF;
+#pragma unroll
+            for (int lvl = 0; lvl < L; ++lvl) digits[(((size_t)g * 2 + q) * L + lvl) * N + j] = digit<BG>(uu, lvl);
+        }
+}
+
+// natural frequency index of lane-order slot (lane, s, k3)
+template <int R1>
+__device__ __forceinline__ int lane_order_k(int lane, int s, int k3) {
+    return (lane & 7) + 8 * s + R1 * (lane >> 3) + 8 * R1 * k3;
+}
+
+template <int N>
+__global__ void __launch_bounds__(64) k_fft_test64(double* __restrict__ out, const void* __restrict__ in, int inverse,
+                                                   const C2<double>* __restrict__ t1, const C2<double>* __restrict__ t2,
+                                                   const C2<double>* __restrict__ wa64) {
+    constexpr int M = N / 2, R1 = M / 64;
+    __shared__ C2<double> s_buf[M];
+    const int lane = threadIdx.x;
+    const size_t g = blockIdx.x;
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, t1, t2, lane);
+    C2<double> v[R1];
+    if (!inverse) {
+        const int32_t* p = (const int32_t*)in + g * N;
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa)
+            v[aa] = cmul(C2<double>{(double)p[lane + 64 * aa], (double)p[lane + 64 * aa + M]}, wa64[aa]);
+        fft_fwd<R1>(v, tw, s_buf, lane);
+        double* o = out + g * N;
+#pragma unroll
+        for (int s = 0; s < R1 / 8; ++s)
+#pragma unroll
+            for (int k3 = 0; k3 < 8; ++k3) {
+                const int k = lane_order_k<R1>(lane, s, k3);
+                o[2 * k] = v[s * 8 + k3].x;
+                o[2 * k + 1] = v[s * 8 + k3].y;
+            }
+    } else {
+        const double* F = (const double*)in + g * N;
+#pragma unroll
+        for (int s = 0; s < R1 / 8; ++s)
+#pragma unroll
+            for (int k3 = 0; k3 < 8; ++k3) {
+                const int k = lane_order_k<R1>(lane, s, k3);
+                v[s * 8 + k3] = C2<double>{F[2 * k], F[2 * k + 1]};
+            }
+        fft_inv<R1>(v, tw, s_buf, lane);
+        double* o = out + g * N;
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            C2<double> z = cmulc(v[aa], wa64[aa]);
+            o[lane + 64 * aa] = z.x / M;
+            o[lane + 64 * aa + M] = z.y / M;
+        }
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, const void* __restrict__ in, int inverse,
+                                                   const C2<float>* __restrict__ t1, const C2<float>* __restrict__ t2,
+                                                   const C2<double>* __restrict__ twist64) {
+    constexpr int M = N / 2, R1 = M / 64;
+    __shared__ C2<float> s_buf[M];
+    const int lane = threadIdx.x;
+    const size_t g = blockIdx.x;
+    LaneTw<float, R1> tw;
+    load_lane_tw(tw, t1, t2, lane);
+    C2<float> v[R1];
+    if (!inverse) {
+        const int32_t* p = (const int32_t*)in + g * N;
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            C2<double> z = cmul(C2<double>{(double)p[lane + 64 * aa], (double)p[lane + 64 * aa + M]},
+                                twist64[lane * R1 + aa]);
+            v[aa] = C2<float>{(float)z.x, (float)z.y};
+        }
+        fft_fwd<R1>(v, tw, s_buf, lane);
+        double* o = out + g * N;
+#pragma unroll
+        for (int s = 0; s < R1 / 8; ++s)
+#pragma unroll
+            for (int k3 = 0; k3 < 8; ++k3) {
+                const int k = lane_order_k<R1>(lane, s, k3);
+                o[2 * k] = v[s * 8 + k3].x;
+                o[2 * k + 1] = v[s * 8 + k3].y;
+            }
+    } else {
+        const double* F = (const double*)in + g * N;
+#pragma unroll
+        for (int s = 0; s < R1 / 8; ++s)
+#pragma unroll
+            for (int k3 = 0; k3 < 8; ++k3) {
+                const int k = lane_order_k<R1>(lane, s, k3);
+                v[s * 8 + k3] = C2<float>{(float)F[2 * k], (float)F[2 * k + 1]};
+            }
+        fft_inv<R1>(v, tw, s_buf, lane);
+        double* o = out + g * N;
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            C2<double> z = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, twist64[lane * R1 + aa]);
+            o[lane + 64 * aa] = z.x / M;
+            o[lane + 64 * aa + M] = z.y / M;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_decrypt(int32_t* __restrict__ bits, const int32_t* __restrict__ s,
+                                                const int32_t* __restrict__ key, int n) {
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const int32_t* x = s + (size_t)g * (n + 1);
+    uint32_t acc = 0;
+    for (int i = lane; i < n; i += 64) acc += (uint32_t)x[i] * (uint32_t)key[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += (uint32_t)__shfl_xor((int)acc, off, 64);
+    if (lane == 0) bits[g] = ((int32_t)((uint32_t)x[n] - acc) > 0) ? 1 : 0;
+}
+
+// =============================================================================================
+// host side: twiddle tables and launchers
+// =============================================================================================
+
+static long double pi_l() { return 3.14159265358979323846264338327950288L; }
+
+// e^{i pi num / N} with the exponent reduced mod 2N before scaling
+static void cis_pi_over_N(long long num, int N, double* re, double* im) {
+    long long e = num % (2LL * N);
+    if (e < 0) e += 2LL * N;
+    long double ang = pi_l() * (long double)e / (long double)N;
+    *re = (double)cosl(ang);
+    *im = (double)sinl(ang);
+}
+
+hipError_t build_twiddles(DevTwiddles& tw, int N) {
+    const int M = N / 2, R1 = M / 64;
+    std::vector<double> t1f(2 * 64 * R1), t2(2 * 64 * 8), tws(2 * 64 * R1), wa(2 * R1);
+    std::vector<float> t1p(2 * 64 * R1), t2f(2 * 64 * 8);
+    for (int L = 0; L < 64; ++L) {
+        for (int k1 = 0; k1 < R1; ++k1) {
+            double re, im;
+            cis_pi_over_N((long long)L * (1 + 4 * k1), N, &re, &im);     // w^{L(1+4k1)}
+            t1f[2 * (L * R1 + k1)] = re; t1f[2 * (L * R1 + k1) + 1] = im;
+            cis_pi_over_N((long long)4 * L * k1, N, &re, &im);           // W_M^{L k1} = w^{4 L k1}
+            t1p[2 * (L * R1 + k1)] = (float)re; t1p[2 * (L * R1 + k1) + 1] = (float)im;
+            cis_pi_over_N((long long)64 * k1 + L, N, &re, &im);          // w^{64a + L}, a = k1
+            tws[2 * (L * R1 + k1)] = re; tws[2 * (L * R1 + k1) + 1] = im;
+        }
+        for (int k2 = 0; k2 < 8; ++k2) {
+            // W_64^{c k2} = w^{(2N/64) c k2}
+            double re, im;
+            cis_pi_over_N((long long)(2 * N / 64) * (L & 7) * k2, N, &re, &im);
+            t2[2 * (L * 8 + k2)] = re; t2[2 * (L * 8 + k2) + 1] = im;
+            t2f[2 * (L * 8 + k2)] = (float)re; t2f[2 * (L * 8 + k2) + 1] = (float)im;
+        }
+    }
+    for (int a = 0; a < R1; ++a) {
+        double re, im;
+        cis_pi_over_N((long long)64 * a, N, &re, &im);
+        wa[2 * a] = re; wa[2 * a + 1] = im;
+    }
+    hipError_t e;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
+        hipError_t r = hipMalloc(dst, bytes);
+        if (r != hipSuccess) return r;
+        return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    tw.N = N;
+    if ((e = up(&tw.t1_fold64, t1f.data(), t1f.size() * 8)) != hipSuccess) return e;
+    if ((e = up(&tw.t2_64, t2.data(), t2.size() * 8)) != hipSuccess) return e;
+    if ((e = up(&tw.t1_plain32, t1p.data(), t1p.size() * 4)) != hipSuccess) return e;
+    if ((e = up(&tw.t2_32, t2f.data(), t2f.size() * 4)) != hipSuccess) return e;
+    if ((e = up(&tw.twist64, tws.data(), tws.size() * 8)) != hipSuccess) return e;
+    if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+void free_twiddles(DevTwiddles& tw) {
+    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist64, tw.wa64};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    tw = DevTwiddles();
+}
+
+bool params_supported(const DevParams& p) {
+    const bool ring = (p.N == 1024 && p.l == 3 && p.Bgbit == 7) || (p.N == 2048 && p.l == 3 && p.Bgbit == 8);
+    return ring && p.n > 0 && p.n <= kMaxLweN && p.ks_basebit == 2 && (p.ks_t == 8 || p.ks_t == 10);
+}
+
+template <int N, int L, int BG>
+static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
+                                hipStream_t st) {
+    BootKArgs A;
+    A.L = Lc;
+    A.n = key.p.n;
+    A.split_bits = key.split_bits;
+    A.wa64 = tw.wa64;
+    A.twist64 = tw.twist64;
+    if (prec == PREC_FP64) {
+        A.bk = key.bk64;
+        A.t1 = tw.t1_fold64;
+        A.t2 = tw.t2_64;
+        hipLaunchKernelGGL((k_boot64<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
+    } else {
+        A.bk = key.bk32;
+        A.t1 = tw.t1_plain32;
+        A.t2 = tw.t2_32;
+        hipLaunchKernelGGL((k_boot32<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
+                            hipStream_t st) {
+    if (Lc.count <= 0) return hipSuccess;
+    if ((prec == PREC_FP64 && !key.bk64) || (prec == PREC_SPLIT32 && !key.bk32)) return hipErrorInvalidValue;
+    if (key.p.N == 1024) return launch_boot_t<1024, 3, 7>(key, tw, prec, Lc, st);
+    if (key.p.N == 2048) return launch_boot_t<2048, 3, 8>(key, tw, prec, Lc, st);
+    return hipErrorInvalidValue;
+}
+
+template <int TKS>
+static hipError_t launch_ks_t(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
+                              int32_t add_b, int count, hipStream_t st) {
+    constexpr int G = 32;
+    const int kN = key.p.N;
+    dim3 grid(key.ksk_np / 64, (count + G - 1) / G);
+    hipLaunchKernelGGL((k_keyswitch<TKS, G>), grid, dim3(64), 0, st, u1, u2, add_b, key.ksk, out, count, kN,
+                       key.p.n, key.ksk_np);
+    return hipGetLastError();
+}
+
+hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2, int32_t add_b,
+                            int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (key.p.ks_t == 8) return launch_ks_t<8>(key, out, u1, u2, add_b, count, st);
+    if (key.p.ks_t == 10) return launch_ks_t<10>(key, out, u1, u2, add_b, count, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32_t* d_bk, void* bk64, void* bk32,
+                          int split_bits, hipStream_t st) {
+    const int npoly = p.n * 2 * p.l * 2;
+    if (p.N == 1024)
+        hipLaunchKernelGGL((k_keyprep<1024>), dim3(npoly), dim3(64), 0, st, d_bk, (C2<double>*)bk64, (float4*)bk32,
+                           split_bits, (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64,
+                           (const C2<double>*)tw.wa64);
+    else if (p.N == 2048)
+        hipLaunchKernelGGL((k_keyprep<2048>), dim3(npoly), dim3(64), 0, st, d_bk, (C2<double>*)bk64, (float4*)bk32,
+                           split_bits, (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64,
+                           (const C2<double>*)tw.wa64);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_ksk_relayout(const DevParams& p, const int32_t* in, int32_t* out, int np, hipStream_t st) {
+    hipLaunchKernelGGL(k_ksk_relayout, dim3(4096), dim3(256), 0, st, in, out, p.N, p.ks_t, p.n, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract(int N, int32_t* u, const int32_t* acc, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (N == 1024) hipLaunchKernelGGL((k_extract<1024>), dim3(count), dim3(256), 0, st, u, acc, count);
+    else if (N == 2048) hipLaunchKernelGGL((k_extract<2048>), dim3(count), dim3(256), 0, st, u, acc, count);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* digits, const int32_t* acc,
+                                   const int32_t* bara, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (p.N == 1024) hipLaunchKernelGGL((k_rotdec<1024, 3, 7>), dim3(count), dim3(64), 0, st, rot, digits, acc, bara);
+    else if (p.N == 2048) hipLaunchKernelGGL((k_rotdec<2048, 3, 8>), dim3(count), dim3(64), 0, st, rot, digits, acc, bara);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse, double* out, const void* in, int count,
+                           hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (prec == PREC_FP64) {
+        if (N == 1024)
+            hipLaunchKernelGGL((k_fft_test64<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
+                               (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
+        else if (N == 2048)
+            hipLaunchKernelGGL((k_fft_test64<2048>), dim3(count), dim3(64), 0, st, out, in, inverse,
+                               (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
+        else return hipErrorInvalidValue;
+    } else {
+        if (N == 1024)
+            hipLaunchKernelGGL((k_fft_test32<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
+                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<double>*)tw.twist64);
+        else if (N == 2048)
+            hipLaunchKernelGGL((k_fft_test32<2048>), dim3(count), dim3(64), 0, st, out, in, inverse,
+                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<double>*)tw.twist64);
+        else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_decrypt(int32_t* bits, const int32_t* samples, const int32_t* key, int n, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_decrypt, dim3(count), dim3(64), 0, st, bits, samples, key, n);
+    return hipGetLastError();
+}
+
+}  // namespace tfhe

@@ -1,0 +1,268 @@
+// tfhe_device.hpp -- device building blocks of the gfx950 bootstrap engine.
+//
+// Folded negacyclic transform.  A real polynomial p of degree < N (mod X^N + 1) is represented
+// by its values at the M = N/2 points w^(4k+1), w = e^{i pi / N}:
+//     z_j = (p_j + i p_{j+M}) w^j,      F_k = sum_{j<M} z_j e^{+2 pi i jk / M}.
+// These are the odd powers of w that the reference's 2N-point embedding evaluates
+// (kernel/src/tfhe_blindRotate_FFT/tGswFFTExternMulToTLwe.cpp:97-130), up to an exact
+// permutation and conjugation, at a quarter of the transform size.
+//
+// One 64-lane wavefront owns one transform of M = 64 * R1 points (R1 = 8 for N = 1024,
+// 16 for N = 2048).  Index split j = 64 a + 8 b + c and k = k1 + R1 k2 + 8 R1 k3 gives three
+// register-resident passes (radix R1 over a, radix 8 over b, radix 8 over c) with two LDS
+// exchanges between them:
+//   pass 1  lane (b,c) = 8b+c, registers a -> k1, then twiddle W_M^{(8b+c) k1}
+//   pass 2  lane (c,k1) = c + 8(k1 mod 8), registers b -> k2, then twiddle W_64^{c k2}
+//   pass 3  lane (k1,k2) = (k1 mod 8) + 8 k2, registers c -> k3
+// The output stays in this "lane order" (for R1 = 8 it is natural order: lane L, register r
+// holds F_{L + 64 r}); the key is stored in the same order, so the pointwise product needs no
+// permutation, and the inverse runs the passes backwards from lane order to natural j order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfhe {
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> operator+(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> operator-(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+// a * b
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+    return {__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ C2<float> cmul(C2<float> a, C2<float> b) {
+    return {__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x)};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ C2<T> cmulc(C2<T> a, C2<T> b) {
+    return {__builtin_fma(a.x, b.x, a.y * b.y), __builtin_fma(a.y, b.x, -(a.x * b.y))};
+}
+__device__ __forceinline__ C2<float> cmulc(C2<float> a, C2<float> b) {
+    return {__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y))};
+}
+// acc += a * b
+template <typename T>
+__device__ __forceinline__ void cmac(C2<T>& acc, C2<T> a, C2<T> b) {
+    acc.x = __builtin_fma(a.x, b.x, acc.x);
+    acc.x = __builtin_fma(-a.y, b.y, acc.x);
+    acc.y = __builtin_fma(a.x, b.y, acc.y);
+    acc.y = __builtin_fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void cmac(C2<float>& acc, C2<float> a, C2<float> b) {
+    acc.x = __builtin_fmaf(a.x, b.x, acc.x);
+    acc.x = __builtin_fmaf(-a.y, b.y, acc.x);
+    acc.y = __builtin_fmaf(a.x, b.y, acc.y);
+    acc.y = __builtin_fmaf(a.y, b.x, acc.y);
+}
+
+// multiply by e^{S i pi/2}: S=+1 -> i, S=-1 -> -i
+template <int S, typename T>
+__device__ __forceinline__ C2<T> mul_i(C2<T> a) {
+    if constexpr (S > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+// multiply by e^{S i pi/4}
+template <int S, typename T>
+__device__ __forceinline__ C2<T> mul_w8(C2<T> a) {
+    const T r = (T)0.70710678118654752440084436210485;
+    if constexpr (S > 0) return {(a.x - a.y) * r, (a.x + a.y) * r};
+    else return {(a.x + a.y) * r, (a.y - a.x) * r};
+}
+// multiply by e^{3 S i pi/4}
+template <int S, typename T>
+__device__ __forceinline__ C2<T> mul_w8_3(C2<T> a) {
+    const T r = (T)0.70710678118654752440084436210485;
+    if constexpr (S > 0) return {(-a.x - a.y) * r, (a.x - a.y) * r};
+    else return {(a.y - a.x) * r, (-a.x - a.y) * r};
+}
+
+// In-place 8-point DFT, X_k = sum_n x_n e^{S 2 pi i nk/8}, natural order in and out.
+template <int S, typename T>
+__device__ __forceinline__ void dft8(C2<T>* v) {
+    C2<T> a0 = v[0] + v[4], a4 = v[0] - v[4];
+    C2<T> a1 = v[1] + v[5], a5 = mul_w8<S>(v[1] - v[5]);
+    C2<T> a2 = v[2] + v[6], a6 = mul_i<S>(v[2] - v[6]);
+    C2<T> a3 = v[3] + v[7], a7 = mul_w8_3<S>(v[3] - v[7]);
+    C2<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = mul_i<S>(a1 - a3);
+    C2<T> b4 = a4 + a6, b6 = a4 - a6, b5 = a5 + a7, b7 = mul_i<S>(a5 - a7);
+    v[0] = b0 + b1; v[4] = b0 - b1;
+    v[2] = b2 + b3; v[6] = b2 - b3;
+    v[1] = b4 + b5; v[5] = b4 - b5;
+    v[3] = b6 + b7; v[7] = b6 - b7;
+}
+
+// In-place 16-point DFT: one radix-2 DIF step, then two 8-point DFTs.
+template <int S, typename T>
+__device__ __forceinline__ void dft16(C2<T>* v) {
+    // twiddles e^{S 2 pi i n/16}, n = 0..7
+    const T c1 = (T)0.92387953251128675612818318939679, s1 = (T)0.38268343236508977172845998403040;
+    const T r = (T)0.70710678118654752440084436210485;
+    C2<T> e[8], o[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) { e[n] = v[n] + v[n + 8]; o[n] = v[n] - v[n + 8]; }
+    const T sg = (T)S;
+    o[1] = cmul(o[1], C2<T>{c1, sg * s1});
+    o[2] = cmul(o[2], C2<T>{r, sg * r});
+    o[3] = cmul(o[3], C2<T>{s1, sg * c1});
+    o[4] = mul_i<S>(o[4]);
+    o[5] = cmul(o[5], C2<T>{-s1, sg * c1});
+    o[6] = cmul(o[6], C2<T>{-r, sg * r});
+    o[7] = cmul(o[7], C2<T>{-c1, sg * s1});
+    dft8<S>(e);
+    dft8<S>(o);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) { v[2 * m] = e[m]; v[2 * m + 1] = o[m]; }
+}
+
+template <int R, int S, typename T>
+__device__ __forceinline__ void dftR(C2<T>* v) {
+    if constexpr (R == 8) dft8<S>(v);
+    else dft16<S>(v);
+}
+
+// Per-lane twiddles of one transform size, loaded once per kernel.
+//   t1[k1] = pass-1 twiddle for lane (b,c): W_M^{(8b+c) k1}, times w^{8b+c} when the twist is
+//            folded in (FP64 path)
+//   t2[k2] = pass-2 twiddle for lane (c,k1): W_64^{c k2}
+template <typename T, int R1>
+struct LaneTw {
+    C2<T> t1[R1];
+    C2<T> t2[8];
+};
+
+template <typename T, int R1>
+__device__ __forceinline__ void load_lane_tw(LaneTw<T, R1>& tw, const C2<T>* __restrict__ g_t1,
+                                             const C2<T>* __restrict__ g_t2, int lane) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) tw.t1[r] = g_t1[lane * R1 + r];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tw.t2[r] = g_t2[lane * 8 + r];
+}
+
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// exchange-2 address: element (k1 = k1lo + 8 s, k2, c); conflict-free for writes by lane
+// (c, k1lo) and reads by lane (k1lo, k2) thanks to the c ^ k1lo swizzle
+__device__ __forceinline__ int x2_addr(int s, int k2, int k1lo, int c) {
+    return (s * 8 + k2) * 64 + 8 * k1lo + (c ^ k1lo);
+}
+
+// Forward transform of one wavefront-owned polynomial.  In: v[a] = z_{64a + lane} (already
+// multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
+// lane order.  buf: M elements of LDS.
+template <int R1, typename T>
+__device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>* buf, int lane) {
+    constexpr int G = R1 / 8;
+    dftR<R1, +1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[k1 * 64 + lane] = v[k1];
+    wave_sync();
+    const int c = lane & 7, k1lo = lane >> 3;
+#pragma unroll
+    for (int s = 0; s < G; ++s)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[s * 8 + b] = buf[(k1lo + 8 * s) * 64 + 8 * b + c];
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+        dft8<+1>(v + s * 8);
+#pragma unroll
+        for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(s, k2, k1lo, c)] = v[s * 8 + k2];
+    }
+    wave_sync();
+    {
+        const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+        for (int s = 0; s < G; ++s)
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) v[s * 8 + cc] = buf[x2_addr(s, k2, k1lo3, cc)];
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < G; ++s) dft8<+1>(v + s * 8);
+}
+
+// Inverse of fft_fwd without the 1/M scale and without the a-dependent untwist constants:
+// in: v in lane order; out: v[a] = M * z_{64a+lane} * (conj of the forward a-constant).
+template <int R1, typename T>
+__device__ __forceinline__ void fft_inv(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>* buf, int lane) {
+    constexpr int G = R1 / 8;
+#pragma unroll
+    for (int s = 0; s < G; ++s) dft8<-1>(v + s * 8);
+    {
+        const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+        for (int s = 0; s < G; ++s)
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) buf[x2_addr(s, k2, k1lo3, cc)] = v[s * 8 + cc];
+    }
+    wave_sync();
+    const int c = lane & 7, k1lo = lane >> 3;
+#pragma unroll
+    for (int s = 0; s < G; ++s)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) v[s * 8 + k2] = buf[x2_addr(s, k2, k1lo, c)];
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < G; ++s) {
+#pragma unroll
+        for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
+        dft8<-1>(v + s * 8);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) buf[(k1lo + 8 * s) * 64 + 8 * b + c] = v[s * 8 + b];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[k1 * 64 + lane];
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
+    dftR<R1, -1>(v);
+}
+
+// ---- integer stages ----
+
+// coefficient j of (X^a - 1) * p, a in [0, 2N): e = (j - a) mod 2N; (X^a p)_j = p_e or -p_{e-N}
+// (kernel/src/tfhe_blindRotate_FFT/tfhe_MuxRotate_FFT.cpp:5-24)
+template <int N>
+__device__ __forceinline__ uint32_t rotdiff(const int32_t* p, int j, int a) {
+    int e = (j - a) & (2 * N - 1);
+    uint32_t s = (uint32_t)p[e & (N - 1)];
+    s = (e < N) ? s : (0u - s);
+    return s - (uint32_t)p[j];
+}
+
+// signed gadget digit of level lvl (tGswTorus32PolynomialDecompH.cpp:29-44) of u = x + offset
+template <int BG>
+__device__ __forceinline__ int digit(uint32_t u_with_offset, int lvl) {
+    return (int)((u_with_offset >> (32 - (lvl + 1) * BG)) & ((1u << BG) - 1)) - (1 << (BG - 1));
+}
+
+template <int L, int BG>
+__host__ __device__ constexpr uint32_t decomp_offset() {
+    uint32_t off = 0;
+    for (int p = 0; p < L; ++p) off += (1u << (BG - 1)) * (1u << (32 - (p + 1) * BG));
+    return off;
+}
+
+// round an integer-valued double (|x| < 2^51) to nearest and reduce mod 2^32:
+// adding 1.5 * 2^52 puts the unit in the last mantissa bit, so the low word is the residue
+__device__ __forceinline__ uint32_t round_mod32(double x) {
+    double y = x + 6755399441055744.0;
+    return (uint32_t)__double_as_longlong(y);
+}
+
+}  // namespace tfhe

@@ -1,0 +1,365 @@
+"""Python binding of libtfhe_mi355x (ctypes over the C ABI in include/tfhe_mi355x.h).
+
+The library is the product: HIP kernels for gfx950 behind the upstream tfhe C API and a flat
+batched ABI.  This module only marshals numpy arrays and device pointers; it never falls back
+to a CPU implementation -- a missing library or device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtfhe_mi355x.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "tfhe_mi355x.h")
+
+PREC_FP64, PREC_SPLIT32 = 0, 1
+PREC_MASK_FP64, PREC_MASK_SPLIT32 = 1, 2
+GATES = {"NAND": 0, "AND": 1, "OR": 2, "XOR": 3, "XNOR": 4, "NOR": 5,
+         "ANDNY": 6, "ANDYN": 7, "ORNY": 8, "ORYN": 9, "MUX": 10}
+MU = 1 << 29  # modSwitchToTorus32(1, 8)
+
+
+class TfheError(RuntimeError):
+    pass
+
+
+class FlatParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("N", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("l", ctypes.c_int32), ("Bgbit", ctypes.c_int32), ("ks_t", ctypes.c_int32),
+                ("ks_basebit", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LweSample(ctypes.Structure):
+    _fields_ = [("a", ctypes.POINTER(ctypes.c_int32)), ("b", ctypes.c_int32),
+                ("current_variance", ctypes.c_double)]
+
+
+_lib = None
+P = ctypes.POINTER
+_i32p, _f64p, _vp = P(ctypes.c_int32), P(ctypes.c_double), ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TfheError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    i32 = ctypes.c_int32
+    sig = {
+        # upstream API
+        "tfhe_random_generator_setSeed": (None, [P(ctypes.c_uint32), i32]),
+        "new_default_gate_bootstrapping_parameters": (_vp, [i32]),
+        "new_gate_bootstrapping_parameters_N2048": (_vp, []),
+        "delete_gate_bootstrapping_parameters": (None, [_vp]),
+        "new_random_gate_bootstrapping_secret_keyset": (_vp, [_vp]),
+        "delete_gate_bootstrapping_secret_keyset": (None, [_vp]),
+        "new_gate_bootstrapping_ciphertext_array": (P(LweSample), [i32, _vp]),
+        "delete_gate_bootstrapping_ciphertext_array": (None, [i32, P(LweSample)]),
+        "bootsSymEncrypt": (None, [P(LweSample), i32, _vp]),
+        "bootsSymDecrypt": (i32, [P(LweSample), _vp]),
+        "bootsCONSTANT": (None, [P(LweSample), i32, _vp]),
+        "bootsNOT": (None, [P(LweSample), P(LweSample), _vp]),
+        "bootsMUX": (None, [P(LweSample), P(LweSample), P(LweSample), P(LweSample), _vp]),
+        "modSwitchToTorus32": (i32, [i32, i32]),
+        "modSwitchFromTorus32": (i32, [i32, i32]),
+        "approxPhase": (i32, [i32, i32]),
+        "dtot32": (i32, [ctypes.c_double]),
+        "tfhe_keyset_export_flat": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
+        "tfhe_flat_params_of": (ctypes.c_int, [_vp, P(FlatParams)]),
+        "tfhe_encrypt_bits_flat": (ctypes.c_int, [_vp, _i32p, i32, _i32p]),
+        "tfhe_decrypt_bits_flat": (ctypes.c_int, [_vp, _i32p, i32, _i32p]),
+        # flat ABI
+        "tfhe_gpu_last_error": (ctypes.c_char_p, []),
+        "tfhe_gpu_init": (ctypes.c_int, [i32]),
+        "tfhe_gpu_device_info": (ctypes.c_int, [ctypes.c_char_p, i32, P(i32), P(ctypes.c_int64)]),
+        "tfhe_gpu_sync": (ctypes.c_int, []),
+        "tfhe_gpu_set_split_bits": (ctypes.c_int, [i32]),
+        "tfhe_gpu_set_default_precision": (ctypes.c_int, [i32]),
+        "tfhe_gpu_key_create": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32, P(_vp)]),
+        "tfhe_gpu_key_upload": (ctypes.c_int, [_vp, i32, P(_vp)]),
+        "tfhe_gpu_key_free": (ctypes.c_int, [_vp]),
+        "tfhe_gpu_key_params": (ctypes.c_int, [_vp, P(FlatParams)]),
+        "tfhe_gpu_gate_batch": (ctypes.c_int, [_vp, i32, i32, _i32p, _i32p, _i32p, _i32p, i32]),
+        "tfhe_gpu_gate_batch_dev": (ctypes.c_int, [_vp, i32, i32, _vp, _vp, _vp, _vp, i32]),
+        "tfhe_gpu_boots_batch": (ctypes.c_int, [i32, P(LweSample), P(LweSample), P(LweSample),
+                                                P(LweSample), i32, _vp]),
+        "tfhe_gpu_bootstrap_woks_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, i32, i32]),
+        "tfhe_gpu_bootstrap_woks_acc_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, _i32p, i32, i32]),
+        "tfhe_gpu_bootstrap_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, i32, i32]),
+        "tfhe_gpu_blind_rotate_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, i32]),
+        "tfhe_gpu_keyswitch_batch": (ctypes.c_int, [_vp, _i32p, _i32p, i32]),
+        "tfhe_gpu_extract_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32]),
+        "tfhe_gpu_extern_product_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, _f64p, i32]),
+        "tfhe_gpu_rotate_decompose_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, _i32p, _i32p, i32]),
+        "tfhe_gpu_fft_forward_batch": (ctypes.c_int, [i32, i32, _f64p, _i32p, i32]),
+        "tfhe_gpu_fft_inverse_batch": (ctypes.c_int, [i32, i32, _f64p, _f64p, i32]),
+        "tfhe_gpu_malloc": (ctypes.c_int, [P(_vp), ctypes.c_size_t]),
+        "tfhe_gpu_free": (ctypes.c_int, [_vp]),
+        "tfhe_gpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+        "tfhe_gpu_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+        "tfhe_gpu_decrypt_batch_dev": (ctypes.c_int, [_i32p, _vp, _i32p, i32, i32]),
+        "tfhe_gpu_profile_enable": (ctypes.c_int, [i32]),
+        "tfhe_gpu_profile_reset": (ctypes.c_int, []),
+        "tfhe_gpu_profile_read": (ctypes.c_int, [ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ck(rc, what=""):
+    if rc != 0:
+        msg = lib().tfhe_gpu_last_error()
+        raise TfheError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def i32p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_i32p)
+
+
+def f64p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_f64p)
+
+
+# ---------------------------------------------------------------------------------------------
+# host-side key sets (upstream API)
+# ---------------------------------------------------------------------------------------------
+class KeySet:
+    """A secret key set made by new_random_gate_bootstrapping_secret_keyset, with flat views."""
+
+    def __init__(self, ring=1024, seed=(0,)):
+        L = lib()
+        self.params_ptr = (L.new_default_gate_bootstrapping_parameters(110) if ring == 1024
+                           else L.new_gate_bootstrapping_parameters_N2048())
+        s = (ctypes.c_uint32 * len(seed))(*seed)
+        L.tfhe_random_generator_setSeed(s, len(seed))
+        self.ptr = L.new_random_gate_bootstrapping_secret_keyset(self.params_ptr)
+        fp = FlatParams()
+        _ck(L.tfhe_flat_params_of(self.params_ptr, ctypes.byref(fp)), "tfhe_flat_params_of")
+        self.fp = fp
+        self.p = fp.as_dict()
+        n, N, k, l, t, bb = fp.n, fp.N, fp.k, fp.l, fp.ks_t, fp.ks_basebit
+        self.lwe_key = np.zeros(n, np.int32)
+        self.tlwe_key = np.zeros(k * N, np.int32)
+        self.bk = np.zeros((n, (k + 1) * l, k + 1, N), np.int32)
+        self.ksk = np.zeros((k * N, t, 1 << bb, n + 1), np.int32)
+        _ck(L.tfhe_keyset_export_flat(self.ptr, i32p(self.lwe_key), i32p(self.tlwe_key), i32p(self.bk),
+                                      i32p(self.ksk)), "tfhe_keyset_export_flat")
+
+    @property
+    def cloud_ptr(self):
+        # TFheGateBootstrappingSecretKeySet: params, lwe_key, tgsw_key, then the embedded cloud set
+        return self.ptr + 3 * ctypes.sizeof(ctypes.c_void_p)
+
+    def encrypt(self, bits):
+        bits = np.ascontiguousarray(np.asarray(bits).reshape(-1), dtype=np.int32)
+        out = np.zeros((len(bits), self.fp.n + 1), np.int32)
+        _ck(lib().tfhe_encrypt_bits_flat(self.ptr, i32p(bits), len(bits), i32p(out)), "encrypt")
+        return out
+
+    def decrypt(self, samples):
+        samples = np.ascontiguousarray(samples, dtype=np.int32)
+        bits = np.zeros(samples.shape[0], np.int32)
+        _ck(lib().tfhe_decrypt_bits_flat(self.ptr, i32p(samples), samples.shape[0], i32p(bits)), "decrypt")
+        return bits
+
+    def free(self):
+        if self.ptr:
+            lib().delete_gate_bootstrapping_secret_keyset(self.ptr)
+            lib().delete_gate_bootstrapping_parameters(self.params_ptr)
+            self.ptr = None
+
+
+# ---------------------------------------------------------------------------------------------
+# device
+# ---------------------------------------------------------------------------------------------
+def init(device=0):
+    _ck(lib().tfhe_gpu_init(device), "tfhe_gpu_init")
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int32()
+    mem = ctypes.c_int64()
+    _ck(lib().tfhe_gpu_device_info(name, 256, ctypes.byref(cu), ctypes.byref(mem)), "device_info")
+    return {"name": name.value.decode(), "cu": cu.value, "hbm_bytes": mem.value}
+
+
+def sync():
+    _ck(lib().tfhe_gpu_sync(), "sync")
+
+
+def set_split_bits(b):
+    _ck(lib().tfhe_gpu_set_split_bits(int(b)), "set_split_bits")
+
+
+class DeviceKey:
+    def __init__(self, flat_params, bk, ksk, prec_mask=PREC_MASK_FP64 | PREC_MASK_SPLIT32):
+        fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+        self.fp = fp
+        self.ptr = ctypes.c_void_p()
+        bk = np.ascontiguousarray(bk, dtype=np.int32)
+        ksk = np.ascontiguousarray(ksk, dtype=np.int32)
+        _ck(lib().tfhe_gpu_key_create(ctypes.byref(fp), i32p(bk), i32p(ksk), prec_mask, ctypes.byref(self.ptr)),
+            "tfhe_gpu_key_create")
+
+    @classmethod
+    def from_keyset(cls, ks: KeySet, prec_mask=PREC_MASK_FP64 | PREC_MASK_SPLIT32):
+        return cls(ks.fp, ks.bk, ks.ksk, prec_mask)
+
+    def free(self):
+        if self.ptr:
+            lib().tfhe_gpu_key_free(self.ptr)
+            self.ptr = None
+
+    # ---- batches on host arrays ----
+    def gate(self, gate, a, b, c=None, prec=PREC_FP64):
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        c = None if c is None else np.ascontiguousarray(c, np.int32)
+        out = np.zeros_like(a)
+        _ck(lib().tfhe_gpu_gate_batch(self.ptr, prec, GATES[gate] if isinstance(gate, str) else gate,
+                                      i32p(out), i32p(a), i32p(b), i32p(c), a.shape[0]), "gate_batch")
+        return out
+
+    def bootstrap_woks(self, x, mu=MU, prec=PREC_FP64, want_acc=False):
+        x = np.ascontiguousarray(x, np.int32)
+        cnt = x.shape[0]
+        u = np.zeros((cnt, self.fp.k * self.fp.N + 1), np.int32)
+        if want_acc:
+            acc = np.zeros((cnt, self.fp.k + 1, self.fp.N), np.int32)
+            _ck(lib().tfhe_gpu_bootstrap_woks_acc_batch(self.ptr, prec, i32p(u), i32p(acc), i32p(x), mu, cnt),
+                "bootstrap_woks_acc")
+            return u, acc
+        _ck(lib().tfhe_gpu_bootstrap_woks_batch(self.ptr, prec, i32p(u), i32p(x), mu, cnt), "bootstrap_woks")
+        return u
+
+    def bootstrap(self, x, mu=MU, prec=PREC_FP64):
+        x = np.ascontiguousarray(x, np.int32)
+        out = np.zeros_like(x)
+        _ck(lib().tfhe_gpu_bootstrap_batch(self.ptr, prec, i32p(out), i32p(x), mu, x.shape[0]), "bootstrap")
+        return out
+
+    def blind_rotate(self, acc, bara, prec=PREC_FP64):
+        acc = np.ascontiguousarray(acc, np.int32).copy()
+        bara = np.ascontiguousarray(bara, np.int32)
+        _ck(lib().tfhe_gpu_blind_rotate_batch(self.ptr, prec, i32p(acc), i32p(bara), acc.shape[0]), "blind_rotate")
+        return acc
+
+    def extern_product(self, acc, row_set, prec=PREC_FP64, want_raw=False):
+        acc = np.ascontiguousarray(acc, np.int32).copy()
+        rs = np.ascontiguousarray(row_set, np.int32)
+        raw = np.zeros(acc.shape, np.float64) if want_raw else None
+        _ck(lib().tfhe_gpu_extern_product_batch(self.ptr, prec, i32p(acc), i32p(rs), f64p(raw), acc.shape[0]),
+            "extern_product")
+        return (acc, raw) if want_raw else acc
+
+    def keyswitch(self, u):
+        u = np.ascontiguousarray(u, np.int32)
+        out = np.zeros((u.shape[0], self.fp.n + 1), np.int32)
+        _ck(lib().tfhe_gpu_keyswitch_batch(self.ptr, i32p(out), i32p(u), u.shape[0]), "keyswitch")
+        return out
+
+    # ---- device-resident batches ----
+    def gate_dev(self, gate, d_out, d_a, d_b, d_c, count, prec=PREC_FP64):
+        _ck(lib().tfhe_gpu_gate_batch_dev(self.ptr, prec, GATES[gate] if isinstance(gate, str) else gate,
+                                          d_out, d_a, d_b, d_c, count), "gate_batch_dev")
+
+
+def extract(flat_params, acc):
+    fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+    acc = np.ascontiguousarray(acc, np.int32)
+    u = np.zeros((acc.shape[0], fp.k * fp.N + 1), np.int32)
+    _ck(lib().tfhe_gpu_extract_batch(ctypes.byref(fp), i32p(u), i32p(acc), acc.shape[0]), "extract")
+    return u
+
+
+def rotate_decompose(flat_params, acc, bara):
+    fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+    acc = np.ascontiguousarray(acc, np.int32)
+    bara = np.ascontiguousarray(bara, np.int32)
+    cnt = acc.shape[0]
+    rot = np.zeros((cnt, 2, fp.N), np.int32)
+    dig = np.zeros((cnt, 2 * fp.l, fp.N), np.int32)
+    _ck(lib().tfhe_gpu_rotate_decompose_batch(ctypes.byref(fp), i32p(rot), i32p(dig), i32p(acc), i32p(bara), cnt),
+        "rotate_decompose")
+    return rot, dig
+
+
+def fft_forward(polys, prec=PREC_FP64):
+    polys = np.ascontiguousarray(polys, np.int32)
+    cnt, N = polys.shape
+    out = np.zeros((cnt, N), np.float64)
+    _ck(lib().tfhe_gpu_fft_forward_batch(N, prec, f64p(out), i32p(polys), cnt), "fft_forward")
+    return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+def fft_inverse(F, prec=PREC_FP64):
+    F = np.asarray(F, np.complex128)
+    cnt, M = F.shape
+    inp = np.ascontiguousarray(np.stack([F.real, F.imag], axis=-1).reshape(cnt, 2 * M))
+    out = np.zeros((cnt, 2 * M), np.float64)
+    _ck(lib().tfhe_gpu_fft_inverse_batch(2 * M, prec, f64p(out), f64p(inp), cnt), "fft_inverse")
+    return out
+
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        _ck(lib().tfhe_gpu_malloc(ctypes.byref(self.ptr), self.nbytes), "malloc")
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        _ck(lib().tfhe_gpu_memcpy_h2d(b.ptr, arr.ctypes.data_as(_vp), arr.nbytes), "h2d")
+        return b
+
+    def to_array(self, shape, dtype=np.int32):
+        out = np.zeros(shape, dtype)
+        _ck(lib().tfhe_gpu_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().tfhe_gpu_free(self.ptr)
+            self.ptr = None
+
+
+def decrypt_dev(d_samples, lwe_key, count):
+    key = np.ascontiguousarray(lwe_key, np.int32)
+    bits = np.zeros(count, np.int32)
+    _ck(lib().tfhe_gpu_decrypt_batch_dev(i32p(bits), d_samples.ptr if isinstance(d_samples, DeviceBuffer)
+                                         else d_samples, i32p(key), key.shape[0], count), "decrypt_dev")
+    return bits
+
+
+def profile_enable(on=True):
+    _ck(lib().tfhe_gpu_profile_enable(1 if on else 0), "profile_enable")
+
+
+def profile_reset():
+    _ck(lib().tfhe_gpu_profile_reset(), "profile_reset")
+
+
+def profile_read(kernel):
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    _ck(lib().tfhe_gpu_profile_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "profile_read")
+    return ms.value, n.value

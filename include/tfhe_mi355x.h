@@ -1,0 +1,374 @@
+/*
+ * tfhe_mi355x.h -- C ABI of the MI355X (gfx950) TFHE gate-bootstrapping engine.
+ *
+ * Two layers share this header:
+ *
+ *  1. The upstream `tfhe` C API that the reference calls (types and prototypes as used at the
+ *     reference call sites cited on each declaration).  A program written against upstream
+ *     tfhe (test_bench/client/alice.c, test_bench/server/cloud.c, test_bench/client/verify.c)
+ *     compiles against this header and links libtfhe_mi355x.so instead.  Gates evaluate on the
+ *     GPU; key generation, encryption and decryption are host code.
+ *
+ *  2. Flat batched entry points (prefix tfhe_gpu_): plain int32 arrays and sizes, used by the
+ *     batch runtime, the benchmark and the parity tests.  They return 0 on success and a
+ *     negative TFHE_E* code on error (message in tfhe_gpu_last_error()).
+ *
+ * Flat layouts (int32 = Torus32, wrap-around mod 2^32):
+ *   LWE sample        [n+1]                 a[0..n-1], b
+ *   extracted sample  [k*N+1]
+ *   TLWE accumulator  [k+1][N]              a[0..k-1], b
+ *   time-domain bk    [n][(k+1)*l][k+1][N]  TGSW rows r = bloc*l + level
+ *   keyswitch key     [k*N][t][2^basebit][n+1]
+ * Supported device parameter sets: k = 1 with (N, l, Bgbit) = (1024, 3, 7) or (2048, 3, 8).
+ */
+#ifndef TFHE_MI355X_H
+#define TFHE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t Torus32;
+
+/* ======================================================================================= */
+/* 1. upstream-compatible types.  Field names and order follow their use in the reference: */
+/*    kernel/src/include/host_to_fpga.h:7-106 and kernel/src/tfhe_blindRotate_FFT/          */
+/*    tfhe_blindRotate_FFT_tb.cpp:244-325.                                                  */
+/* ======================================================================================= */
+
+typedef struct LweParams {          /* host_to_fpga.h:29-33 */
+    int32_t n;
+    double alpha_min;
+    double alpha_max;
+} LweParams;
+
+typedef struct TLweParams {         /* host_to_fpga.h:35-41 */
+    int32_t N;
+    int32_t k;
+    double alpha_min;
+    double alpha_max;
+    LweParams extracted_lweparams;
+} TLweParams;
+
+typedef struct TGswParams {         /* host_to_fpga.h:43-55 */
+    int32_t l;
+    int32_t Bgbit;
+    int32_t Bg;
+    int32_t halfBg;
+    uint32_t maskMod;
+    const TLweParams* tlwe_params;
+    int32_t kpl;
+    Torus32* h;
+    uint32_t offset;
+} TGswParams;
+
+typedef struct TFheGateBootstrappingParameterSet {   /* tb :159, :244, :268 */
+    int32_t ks_t;
+    int32_t ks_basebit;
+    const LweParams* in_out_params;
+    const TGswParams* tgsw_params;
+} TFheGateBootstrappingParameterSet;
+
+typedef struct IntPolynomial { int32_t N; int32_t* coefs; } IntPolynomial;
+typedef struct TorusPolynomial { int32_t N; Torus32* coefsT; } TorusPolynomial;   /* host_to_fpga.h:7-11 */
+/* data: N/2 complex doubles (host_to_fpga.h:13-19).  In this engine the FFT-domain key lives in
+ * device memory; data is NULL and precomp points at the owning device-key record. */
+typedef struct LagrangeHalfCPolynomial { void* data; void* precomp; } LagrangeHalfCPolynomial;
+
+typedef struct LweKey { const LweParams* params; int32_t* key; } LweKey;
+typedef struct TLweKey { const TLweParams* params; IntPolynomial* key; } TLweKey;
+typedef struct TGswKey {
+    const TGswParams* params;
+    const TLweParams* tlwe_params;
+    IntPolynomial* key;
+    TLweKey tlwe_key;
+} TGswKey;
+
+typedef struct LweSample {          /* host_to_fpga.h:21-27 */
+    Torus32* a;
+    Torus32 b;
+    double current_variance;
+} LweSample;
+
+typedef struct TLweSample {         /* tb :97-109, :317 */
+    TorusPolynomial* a;             /* k+1 polynomials */
+    TorusPolynomial* b;             /* alias of a[k]   */
+    double current_variance;
+    int32_t k;
+} TLweSample;
+
+typedef struct TLweSampleFFT {      /* host_to_fpga.h:57-63 */
+    LagrangeHalfCPolynomial* a;
+    LagrangeHalfCPolynomial* b;
+    double current_variance;
+    int32_t k;
+} TLweSampleFFT;
+
+typedef struct TGswSample {
+    TLweSample* all_sample;         /* (k+1)*l rows */
+    TLweSample** bloc_sample;       /* bloc_sample[bloc][i] = all_sample[bloc*l + i] */
+    int32_t k;
+    int32_t l;
+} TGswSample;
+
+typedef struct TGswSampleFFT {      /* host_to_fpga.h:65-72 */
+    TLweSampleFFT* all_samples;
+    TLweSampleFFT** sample;
+    int32_t k;
+    int32_t l;
+} TGswSampleFFT;
+
+typedef struct LweKeySwitchKey {    /* host_to_fpga.h:74-95 */
+    int32_t n;                      /* input dimension k*N */
+    int32_t t;
+    int32_t basebit;
+    int32_t base;
+    const LweParams* out_params;
+    LweSample* ks0_raw;             /* n*t*base samples */
+    LweSample** ks1_raw;
+    LweSample*** ks;                /* ks[i][j][h] */
+} LweKeySwitchKey;
+
+typedef struct LweBootstrappingKey {
+    const LweParams* in_out_params;
+    const TGswParams* bk_params;
+    const TLweParams* accum_params;
+    const LweParams* extract_params;
+    TGswSample* bk;
+    LweKeySwitchKey* ks;
+} LweBootstrappingKey;
+
+typedef struct LweBootstrappingKeyFFT {   /* host_to_fpga.h:97-106 */
+    const LweParams* in_out_params;
+    const TGswParams* bk_params;
+    const TLweParams* accum_params;
+    const LweParams* extract_params;
+    const TGswSampleFFT* bkFFT;
+    const LweKeySwitchKey* ks;
+} LweBootstrappingKeyFFT;
+
+typedef struct TFheGateBootstrappingCloudKeySet {   /* tb :240-244, :268, :293-295 */
+    const TFheGateBootstrappingParameterSet* params;
+    const LweBootstrappingKey* bk;
+    const LweBootstrappingKeyFFT* bkFFT;
+} TFheGateBootstrappingCloudKeySet;
+
+typedef struct TFheGateBootstrappingSecretKeySet {  /* alice.c:15, verify.c:147-151 */
+    const TFheGateBootstrappingParameterSet* params;
+    const LweKey* lwe_key;
+    const TGswKey* tgsw_key;
+    TFheGateBootstrappingCloudKeySet cloud;
+} TFheGateBootstrappingSecretKeySet;
+
+/* ======================================================================================= */
+/* 1a. upstream functions                                                                  */
+/* ======================================================================================= */
+
+/* torus numerics -- kernel/src/numeric-functions.cpp:16-76 */
+void tfhe_random_generator_setSeed(uint32_t* values, int32_t size);      /* :16, tb :164 */
+Torus32 dtot32(double d);                                                 /* :32 */
+double t32tod(Torus32 x);                                                 /* :36 */
+Torus32 gaussian32(Torus32 message, double sigma);                        /* :22 */
+Torus32 approxPhase(Torus32 phase, int32_t Msize);                        /* :45 */
+int32_t modSwitchFromTorus32(Torus32 phase, int32_t Msize);               /* :59 */
+Torus32 modSwitchToTorus32(int32_t mu, int32_t Msize);                    /* :71, tb :267 */
+
+/* parameters (upstream constructors; defaults = kernel/src/include/constants.h:26-36) */
+LweParams* new_LweParams(int32_t n, double alpha_min, double alpha_max);
+void delete_LweParams(LweParams* obj);
+TLweParams* new_TLweParams(int32_t N, int32_t k, double alpha_min, double alpha_max);
+void delete_TLweParams(TLweParams* obj);
+TGswParams* new_TGswParams(int32_t l, int32_t Bgbit, const TLweParams* tlwe_params);
+void delete_TGswParams(TGswParams* obj);
+TFheGateBootstrappingParameterSet* new_TFheGateBootstrappingParameterSet(
+    int32_t ks_t, int32_t ks_basebit, const LweParams* in_out_params, const TGswParams* tgsw_params);
+void delete_TFheGateBootstrappingParameterSet(TFheGateBootstrappingParameterSet* params);
+TFheGateBootstrappingParameterSet* new_default_gate_bootstrapping_parameters(int32_t minimum_lambda); /* alice.c:9 */
+void delete_gate_bootstrapping_parameters(TFheGateBootstrappingParameterSet* params);              /* alice.c:63 */
+/* the N=2048 ring (n=750, l=3, Bgbit=8, t=10, basebit=2, stdev 2^-17 / 2^-30); not an upstream symbol */
+TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters_N2048(void);
+
+/* key sets -- alice.c:15, alice.c:62, cloud.c:137 */
+TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secret_keyset(
+    const TFheGateBootstrappingParameterSet* params);
+void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSecretKeySet* keyset);
+void delete_gate_bootstrapping_cloud_keyset(TFheGateBootstrappingCloudKeySet* keyset);
+
+/* samples -- alice.c:34, alice.c:57, tb :270, tb :315 */
+LweSample* new_LweSample(const LweParams* params);
+LweSample* new_LweSample_array(int32_t nbelts, const LweParams* params);
+void delete_LweSample(LweSample* obj);
+void delete_LweSample_array(int32_t nbelts, LweSample* obj);
+LweSample* new_gate_bootstrapping_ciphertext(const TFheGateBootstrappingParameterSet* params);
+LweSample* new_gate_bootstrapping_ciphertext_array(int32_t nbelems, const TFheGateBootstrappingParameterSet* params);
+void delete_gate_bootstrapping_ciphertext(LweSample* sample);
+void delete_gate_bootstrapping_ciphertext_array(int32_t nbelems, LweSample* samples);
+TLweSample* new_TLweSample(const TLweParams* params);
+void delete_TLweSample(TLweSample* obj);
+TorusPolynomial* new_TorusPolynomial(int32_t N);
+void delete_TorusPolynomial(TorusPolynomial* obj);
+
+/* LWE arithmetic -- kernel/src/lwe-functions.cpp */
+void lweKeyGen(LweKey* result);                                                         /* :21 */
+void lweSymEncrypt(LweSample* result, Torus32 message, double alpha, const LweKey* key); /* :36 */
+Torus32 lwePhase(const LweSample* sample, const LweKey* key);                          /* :72 */
+Torus32 lweSymDecrypt(const LweSample* sample, const LweKey* key, int32_t Msize);       /* :88 */
+void lweClear(LweSample* result, const LweParams* params);                             /* :100 */
+void lweCopy(LweSample* result, const LweSample* sample, const LweParams* params);     /* :110 */
+void lweNegate(LweSample* result, const LweSample* sample, const LweParams* params);   /* :120 */
+void lweNoiselessTrivial(LweSample* result, Torus32 mu, const LweParams* params);      /* :130 */
+void lweAddTo(LweSample* result, const LweSample* sample, const LweParams* params);     /* :139 */
+void lweSubTo(LweSample* result, const LweSample* sample, const LweParams* params);     /* :222 */
+void lweAddMulTo(LweSample* result, int32_t p, const LweSample* sample, const LweParams* params); /* :237 */
+void lweSubMulTo(LweSample* result, int32_t p, const LweSample* sample, const LweParams* params); /* :246 */
+
+/* gate API -- test_bench/server/cloud.c:10-35, alice.c:36, verify.c:27 */
+void bootsSymEncrypt(LweSample* result, int32_t message, const TFheGateBootstrappingSecretKeySet* key);
+int32_t bootsSymDecrypt(const LweSample* sample, const TFheGateBootstrappingSecretKeySet* key);
+void bootsCONSTANT(LweSample* result, int32_t value, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsNOT(LweSample* result, const LweSample* ca, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsCOPY(LweSample* result, const LweSample* ca, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsNAND(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsAND(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsXOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsXNOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsNOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsANDNY(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsANDYN(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsORNY(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsORYN(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+void bootsMUX(LweSample* result, const LweSample* a, const LweSample* b, const LweSample* c,
+              const TFheGateBootstrappingCloudKeySet* bk);
+
+/* bootstrapping internals (GPU) -- tb :280 (tfhe_bootstrap_FFT), tb :325 (tfhe_blindRotate_FFT),
+ * kernel/src/tfhe_bootstrap_woKS_FFT/tfhe_bootstrap_woKS_FFT.cpp:240 (woKS),
+ * kernel/src/lwe-keyswitch-functions.cpp:228 (lweKeySwitch). */
+void tfhe_bootstrap_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
+void tfhe_bootstrap_woKS_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
+void tfhe_blindRotate_FFT(TLweSample* accum, const TGswSampleFFT* bk, const int32_t* bara, int32_t n,
+                          const TGswParams* bk_params);
+void tfhe_blindRotateAndExtract_FFT(LweSample* result, const TorusPolynomial* v, const TGswSampleFFT* bk,
+                                    int32_t barb, const int32_t* bara, int32_t n, const TGswParams* bk_params);
+void lweKeySwitch(LweSample* result, const LweKeySwitchKey* ks, const LweSample* sample);
+void tLweExtractLweSample(LweSample* result, const TLweSample* x, const LweParams* params,
+                          const TLweParams* rparams);
+
+/* ======================================================================================= */
+/* 2. flat batched C ABI                                                                   */
+/* ======================================================================================= */
+
+enum {
+    TFHE_OK = 0,
+    TFHE_E_ARG = -1,          /* bad argument (null pointer, size out of range)        */
+    TFHE_E_UNSUPPORTED = -2,  /* parameter set or precision the device path lacks       */
+    TFHE_E_HIP = -3,          /* HIP runtime error                                       */
+    TFHE_E_NODEV = -4,        /* no GPU / extension not usable                            */
+    TFHE_E_NOMEM = -5
+};
+
+enum { TFHE_PREC_FP64 = 0, TFHE_PREC_SPLIT32 = 1 };
+enum { TFHE_PREC_MASK_FP64 = 1, TFHE_PREC_MASK_SPLIT32 = 2 };
+
+enum {
+    TFHE_GATE_NAND = 0, TFHE_GATE_AND, TFHE_GATE_OR, TFHE_GATE_XOR, TFHE_GATE_XNOR,
+    TFHE_GATE_NOR, TFHE_GATE_ANDNY, TFHE_GATE_ANDYN, TFHE_GATE_ORNY, TFHE_GATE_ORYN,
+    TFHE_GATE_MUX
+};
+
+typedef struct tfhe_flat_params {
+    int32_t n, N, k, l, Bgbit, ks_t, ks_basebit;
+} tfhe_flat_params;
+
+typedef struct tfhe_gpu_key tfhe_gpu_key;   /* opaque device key (FFT-domain bk + KSK) */
+
+const char* tfhe_gpu_last_error(void);
+/* Selects the device and creates the library stream.  Called implicitly by the others. */
+int tfhe_gpu_init(int32_t device);
+int tfhe_gpu_device_info(char* name, int32_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
+int tfhe_gpu_sync(void);
+
+/* Device keys.  Replaces the FPGA flattening of the FFT-domain key
+ * (kernel/src/include/host_to_fpga.h:65-106): the time-domain bk is uploaded and transformed on
+ * the device into a flat [n][(k+1)l][k+1][N/2] array in FP64 (prec mask bit 0) and/or as
+ * FP32 transforms of the hi/lo halves of each key coefficient (bit 1); the KSK is re-laid-out
+ * without its h=0 rows. */
+int tfhe_gpu_key_create(const tfhe_flat_params* p, const int32_t* bk, const int32_t* ksk,
+                        int32_t prec_mask, tfhe_gpu_key** out);
+int tfhe_gpu_key_upload(const TFheGateBootstrappingCloudKeySet* bk, int32_t prec_mask, tfhe_gpu_key** out);
+int tfhe_gpu_key_free(tfhe_gpu_key* key);
+int tfhe_gpu_key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
+/* split point of the SPLIT32 key: key = hi * 2^split + lo.  Applies to keys created afterwards. */
+int tfhe_gpu_set_split_bits(int32_t bits);
+/* precision used by the upstream-API entry points (boots*, tfhe_bootstrap_FFT, ...) */
+int tfhe_gpu_set_default_precision(int32_t prec);
+
+/* flat views of host key sets made by new_random_gate_bootstrapping_secret_keyset (any output may
+ * be NULL): LWE key [n], TLWE key [k][N], time-domain bk, keyswitch key (layouts above) */
+int tfhe_keyset_export_flat(const TFheGateBootstrappingSecretKeySet* key, int32_t* lwe_key, int32_t* tlwe_key,
+                            int32_t* bk, int32_t* ksk);
+int tfhe_flat_params_of(const TFheGateBootstrappingParameterSet* params, tfhe_flat_params* out);
+/* bootsSymEncrypt / bootsSymDecrypt over flat arrays [count][n+1] (same draw order as a loop of
+ * bootsSymEncrypt calls) */
+int tfhe_encrypt_bits_flat(const TFheGateBootstrappingSecretKeySet* key, const int32_t* bits, int32_t count,
+                           int32_t* out);
+int tfhe_decrypt_bits_flat(const TFheGateBootstrappingSecretKeySet* key, const int32_t* samples, int32_t count,
+                           int32_t* bits);
+
+/* Gate batches on host arrays [count][n+1] (gate semantics of upstream boots*, cloud.c:10-35).
+ * a, b: gate inputs; c: third input of MUX (ignored by the other gates).  count may be 0. */
+int tfhe_gpu_gate_batch(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* out, const int32_t* a,
+                        const int32_t* b, const int32_t* c, int32_t count);
+/* The same on device-resident arrays; stream may be NULL (library stream). */
+int tfhe_gpu_gate_batch_dev(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* d_out,
+                            const int32_t* d_a, const int32_t* d_b, const int32_t* d_c, int32_t count);
+/* Batched gates over upstream LweSample arrays (same gate codes). */
+int tfhe_gpu_boots_batch(int32_t gate, LweSample* out, const LweSample* a, const LweSample* b,
+                         const LweSample* c, int32_t count, const TFheGateBootstrappingCloudKeySet* bk);
+
+/* Stage entry points on host arrays (parity tests and the upstream internals above). */
+int tfhe_gpu_bootstrap_woks_batch(tfhe_gpu_key* key, int32_t prec, int32_t* u, const int32_t* x,
+                                  int32_t mu, int32_t count);
+/* as tfhe_gpu_bootstrap_woks_batch, also returning the blind-rotated accumulators [count][k+1][N] */
+int tfhe_gpu_bootstrap_woks_acc_batch(tfhe_gpu_key* key, int32_t prec, int32_t* u, int32_t* acc,
+                                      const int32_t* x, int32_t mu, int32_t count);
+int tfhe_gpu_bootstrap_batch(tfhe_gpu_key* key, int32_t prec, int32_t* out, const int32_t* x,
+                             int32_t mu, int32_t count);
+int tfhe_gpu_blind_rotate_batch(tfhe_gpu_key* key, int32_t prec, int32_t* acc, const int32_t* bara,
+                                int32_t count);
+int tfhe_gpu_keyswitch_batch(tfhe_gpu_key* key, int32_t* out, const int32_t* u, int32_t count);
+int tfhe_gpu_extract_batch(const tfhe_flat_params* p, int32_t* u, const int32_t* acc, int32_t count);
+/* One external product bk[row_set[g]] (.) acc[g] per accumulator.  raw (optional) receives the
+ * pre-rounding values in integer units [count][k+1][N] (FP64). */
+int tfhe_gpu_extern_product_batch(tfhe_gpu_key* key, int32_t prec, int32_t* acc, const int32_t* row_set,
+                                  double* raw, int32_t count);
+/* (X^bara - 1) * acc and its gadget decomposition, as computed inside the bootstrap kernel. */
+int tfhe_gpu_rotate_decompose_batch(const tfhe_flat_params* p, int32_t* rot, int32_t* digits,
+                                    const int32_t* acc, const int32_t* bara, int32_t count);
+/* Folded negacyclic transform of integer polynomials [count][N] -> [count][N/2] complex
+ * (interleaved re, im) in natural order F_k = p(e^{i pi (4k+1)/N}); and its inverse to
+ * integer-unit doubles [count][N]. */
+int tfhe_gpu_fft_forward_batch(int32_t N, int32_t prec, double* out, const int32_t* in, int32_t count);
+int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, const double* in, int32_t count);
+
+/* device memory helpers for the *_dev entry points */
+int tfhe_gpu_malloc(void** ptr, size_t bytes);
+int tfhe_gpu_free(void* ptr);
+int tfhe_gpu_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int tfhe_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* phase > 0 decryption of device samples [count][n+1] into host bits (benchmark audits) */
+int tfhe_gpu_decrypt_batch_dev(int32_t* bits, const int32_t* d_samples, const int32_t* lwe_key,
+                               int32_t n, int32_t count);
+
+/* kernel timing on the library stream (hipEvent pairs around each launch) */
+int tfhe_gpu_profile_enable(int32_t on);
+int tfhe_gpu_profile_reset(void);
+int tfhe_gpu_profile_read(const char* kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

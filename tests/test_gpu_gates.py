@@ -1,0 +1,97 @@
+"""End-to-end gate bootstrapping on the GPU: truth tables under decryption, the reference's
+known answer (OR of 499 and 388 over 16 bits = 503, test_bench/client/alice.c:33-44 and
+verify.c:31), bit-exact FP64 parity with the oracle on identical inputs, and batch edge cases."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+P = O.PARAMS_1024
+
+
+def T():
+    import tfhe_amd
+    return tfhe_amd
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("gate", list(O.GATES))
+def test_gate_truth_table(gpu_env, gate, prec, rng):
+    ks, dk, keys = gpu_env
+    cnt = 96
+    bits = rng.integers(0, 2, (3, cnt)).astype(np.int32)
+    bits[:2, :8] = np.array([[0, 0, 1, 1, 0, 0, 1, 1], [0, 1, 0, 1, 0, 1, 0, 1]])
+    bits[2, :8] = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    ca, cb, cc = (ks.encrypt(bits[i]) for i in range(3))
+    out = dk.gate(gate, ca, cb, cc if gate == "MUX" else None, prec=prec)
+    got = ks.decrypt(out)
+    want = np.array([O.TRUTH[gate](int(a), int(b), int(c)) for a, b, c in bits.T])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("gate", ["NAND", "XOR", "MUX"])
+def test_gate_fp64_bit_exact_vs_oracle(gpu_env, gate, rng):
+    ks, dk, keys = gpu_env
+    bits = rng.integers(0, 2, (3, 3)).astype(np.int32)
+    ca, cb, cc = (ks.encrypt(bits[i]) for i in range(3))
+    out = dk.gate(gate, ca, cb, cc if gate == "MUX" else None, prec=0)
+    for i in range(3):
+        want = O.gate(keys, gate, ca[i], cb[i], cc[i])
+        assert np.array_equal(out[i], want), i
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_known_answer_or_499_388(gpu_env, prec):
+    ks, dk, _ = gpu_env
+    a = np.array([(499 >> i) & 1 for i in range(16)], np.int32)
+    b = np.array([(388 >> i) & 1 for i in range(16)], np.int32)
+    out = dk.gate("OR", ks.encrypt(a), ks.encrypt(b), prec=prec)
+    bits = ks.decrypt(out)
+    assert int(sum(int(v) << i for i, v in enumerate(bits))) == (499 | 388) == 503
+
+
+def test_empty_and_ragged_batches(gpu_env, rng):
+    ks, dk, _ = gpu_env
+    z = np.zeros((0, P.n + 1), np.int32)
+    assert dk.gate("NAND", z, z).shape == (0, P.n + 1)
+    for cnt in (1, 31, 33, 65):
+        bits = rng.integers(0, 2, (2, cnt)).astype(np.int32)
+        out = dk.gate("AND", ks.encrypt(bits[0]), ks.encrypt(bits[1]))
+        assert np.array_equal(ks.decrypt(out), bits[0] & bits[1]), cnt
+
+
+def test_chained_gates_stay_correct(gpu_env, rng):
+    """Outputs feed the next level (bootstrapped noise, not fresh noise)."""
+    ks, dk, _ = gpu_env
+    cnt = 128
+    bits = rng.integers(0, 2, cnt).astype(np.int32)
+    ct = ks.encrypt(bits)
+    for level in range(6):
+        perm = rng.permutation(cnt)
+        out = dk.gate("NAND", ct, ct[perm], prec=level % 2)
+        bits = 1 - (bits & bits[perm])
+        ct = out
+        assert np.array_equal(ks.decrypt(ct), bits), level
+
+
+def test_upstream_api_boots_gates(gpu_env):
+    """bootsNAND/bootsMUX/bootsCONSTANT through the upstream-compatible entry points."""
+    ks, _, _ = gpu_env
+    L = T().lib()
+    arr = L.new_gate_bootstrapping_ciphertext_array(4, ks.params_ptr)
+    try:
+        L.bootsSymEncrypt(ctypes.byref(arr[0]), 1, ks.ptr)
+        L.bootsSymEncrypt(ctypes.byref(arr[1]), 0, ks.ptr)
+        L.bootsCONSTANT(ctypes.byref(arr[2]), 1, ks.cloud_ptr)
+        # MUX(sel=1, b=0, c=1) = 0, written over its own selector input
+        L.bootsMUX(ctypes.byref(arr[3]), ctypes.byref(arr[0]), ctypes.byref(arr[1]), ctypes.byref(arr[2]),
+                   ks.cloud_ptr)
+        assert L.bootsSymDecrypt(ctypes.byref(arr[3]), ks.ptr) == 0
+        L.bootsNAND.argtypes = [ctypes.c_void_p] * 4
+        L.bootsNAND(ctypes.addressof(arr[0]), ctypes.addressof(arr[0]), ctypes.addressof(arr[2]), ks.cloud_ptr)
+        assert L.bootsSymDecrypt(ctypes.byref(arr[0]), ks.ptr) == 0
+    finally:
+        L.delete_gate_bootstrapping_ciphertext_array(4, arr)

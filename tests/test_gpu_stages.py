@@ -1,0 +1,171 @@
+"""GPU parity of the stage kernels against the oracle, through the C ABI.
+
+Bars: bit-exact for the integer stages (rotation, decomposition, extraction, keyswitch) and for
+the FP64 external product (its FFT error stays far below 1/2, so rounding recovers the exact
+int64 product); the split-FP32 external product is held to a stated torus error bound.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P = O.PARAMS_1024
+N, M = P.N, P.N // 2
+FP = dict(n=P.n, N=P.N, k=P.k, l=P.l, Bgbit=P.Bgbit, ks_t=P.ks_t, ks_basebit=P.ks_basebit)
+
+
+def T():
+    import tfhe_amd
+    return tfhe_amd
+
+
+def closed_form_transform(p):
+    """F_k = p(w^(4k+1)), w = e^{i pi/N}: direct evaluation at the roots of X^N + 1."""
+    p = np.asarray(p, dtype=np.float64)
+    n = len(p)
+    j = np.arange(n)
+    k = np.arange(n // 2)
+    e = (j[None, :] * (4 * k + 1)[:, None]) % (2 * n)
+    return (p[None, :] * np.exp(1j * np.pi * e / n)).sum(axis=1)
+
+
+def rand_torus(rng, shape):
+    return rng.integers(-2**31, 2**31, size=shape, dtype=np.int64).astype(np.int32)
+
+
+@pytest.mark.parametrize("prec,rtol", [(0, 1e-12), (1, 4e-7)])
+def test_fft_forward_matches_closed_form(prec, rtol, rng):
+    polys = np.concatenate([rand_torus(rng, (3, N)), rng.integers(-64, 64, (3, N)).astype(np.int32),
+                            np.eye(N, dtype=np.int32)[[0, 1, N // 2, N - 1]]])
+    got = T().fft_forward(polys, prec=prec)
+    for i in range(polys.shape[0]):
+        ref = closed_form_transform(polys[i])
+        err = np.abs(got[i] - ref).max()
+        assert err <= rtol * max(1.0, np.abs(ref).max()) * 8, (i, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-3), (1, None)])
+def test_fft_roundtrip(prec, tol, rng):
+    polys = rng.integers(-64, 64, (4, N)).astype(np.int32)
+    F = T().fft_forward(polys, prec=prec)
+    back = T().fft_inverse(F, prec=prec)
+    err = np.abs(back - polys).max()
+    assert err < (tol if tol is not None else 1e-3 * 64), err
+
+
+def test_rotate_decompose_bit_exact(rng):
+    acc = rand_torus(rng, (12, 2, N))
+    bara = np.array([0, 1, 2, N - 1, N, N + 1, 2 * N - 1, 7, 1000, 1500, 2000, 513], np.int32)
+    rot, dig = T().rotate_decompose(FP, acc, bara)
+    for g in range(len(bara)):
+        for q in range(2):
+            want = O.poly_mul_xai_minus_one(int(bara[g]), acc[g, q])
+            assert np.array_equal(rot[g, q], want), (g, q)
+            wd = O.decompose(want, P.l, P.Bgbit)
+            assert np.array_equal(dig[g, q * P.l:(q + 1) * P.l], wd), (g, q)
+
+
+def test_extract_bit_exact(rng):
+    acc = rand_torus(rng, (5, 2, N))
+    u = T().extract(FP, acc)
+    for g in range(acc.shape[0]):
+        assert np.array_equal(u[g], O.extract(P, acc[g]))
+
+
+def test_extern_product_fp64_exact(gpu_env, rng):
+    _, dk, keys = gpu_env
+    acc = rand_torus(rng, (6, 2, N))
+    rows = np.array([0, 1, 5, 100, 628, 629], np.int32)
+    got, raw = dk.extern_product(acc, rows, prec=0, want_raw=True)
+    worst = 0.0
+    for g in range(acc.shape[0]):
+        want, exact = O.extern_product_exact(P, acc[g], keys.bk[rows[g]], want_raw=True)
+        assert np.array_equal(got[g], want), g
+        worst = max(worst, float(np.abs(raw[g] - exact.astype(np.float64)).max()))
+    print(f"FP64 external product: max |pre-rounding - exact int64| = {worst:.3e} (integer units)")
+    assert worst < 0.25
+
+
+def test_extern_product_split32_error(gpu_env, rng):
+    _, dk, keys = gpu_env
+    acc = rand_torus(rng, (6, 2, N))
+    rows = np.array([0, 3, 17, 300, 500, 629], np.int32)
+    got = dk.extern_product(acc, rows, prec=1)
+    errs = []
+    for g in range(acc.shape[0]):
+        want = O.extern_product_exact(P, acc[g], keys.bk[rows[g]])
+        d = (got[g].astype(np.int64) - want.astype(np.int64) + 2**31) % 2**32 - 2**31
+        errs.append(np.abs(d))
+    e = np.concatenate([x.ravel() for x in errs])
+    print(f"split-FP32 external product torus error: max {e.max()} rms {np.sqrt((e.astype(float)**2).mean()):.1f}"
+          f" (units of 2^-32)")
+    # stated bound: 2^16 units = 2^-16 of the torus per external product
+    assert e.max() < 2**16
+
+
+def test_keyswitch_bit_exact(gpu_env, rng):
+    _, dk, keys = gpu_env
+    u = rand_torus(rng, (40, N + 1))
+    got = dk.keyswitch(u)
+    for g in range(u.shape[0]):
+        assert np.array_equal(got[g], O.keyswitch(keys, u[g])), g
+
+
+def test_blind_rotate_fp64_matches_oracle(gpu_env, rng):
+    _, dk, keys = gpu_env
+    acc = np.stack([O.test_vector(P, int(b), T().MU) for b in (0, 5, 1777)])
+    bara = rng.integers(0, 2 * N, (3, P.n)).astype(np.int32)
+    bara[1, ::3] = 0
+    got = dk.blind_rotate(acc, bara, prec=0)
+    for g in range(acc.shape[0]):
+        want = O.blind_rotate(keys, acc[g], bara[g])
+        assert np.array_equal(got[g], want), g
+
+
+def test_blind_rotate_fp64_matches_exact_int64(gpu_env, rng):
+    _, dk, keys = gpu_env
+    acc = O.test_vector(P, 321, T().MU)[None]
+    bara = rng.integers(0, 2 * N, (1, P.n)).astype(np.int32)
+    got = dk.blind_rotate(acc, bara, prec=0)
+    want = O.blind_rotate(keys, acc[0], bara[0], exact=True)
+    assert np.array_equal(got[0], want)
+
+
+def test_blind_rotate_split32_phase_error(gpu_env, rng):
+    """After an inexact step the mask words diverge (different digits select different key-row
+    combinations), so both paths are measured by the TLWE phase b - a.s against the noiseless
+    rotation X^(sum bara_i s_i) * testvector, which is what decryption sees."""
+    _, dk, keys = gpu_env
+    acc = np.stack([O.test_vector(P, b, T().MU) for b in (99, 1234)])
+    bara = rng.integers(0, 2 * N, (2, P.n)).astype(np.int32)
+    res = {p: dk.blind_rotate(acc, bara, prec=p) for p in (0, 1)}
+    rms = {}
+    worst = {}
+    for p in (0, 1):
+        errs = []
+        for g in range(2):
+            rot = int((bara[g].astype(np.int64) * keys.lwe_key).sum() % (2 * N))
+            ideal = O.poly_mul_xai(rot, acc[g, 1])
+            ph = O.tlwe_phase(P, res[p][g], keys.tlwe_key)
+            errs.append((ph.astype(np.int64) - ideal.astype(np.int64) + 2**31) % 2**32 - 2**31)
+        e = np.abs(np.concatenate(errs)).astype(np.float64)
+        rms[p], worst[p] = np.sqrt((e ** 2).mean()), e.max()
+    print(f"blind rotation phase noise (units of 2^-32): FP64 rms {rms[0]:.3e} max {worst[0]:.3e}; "
+          f"split-FP32 rms {rms[1]:.3e} max {worst[1]:.3e}")
+    # stated bounds: split-FP32 adds at most as much noise again as the key's own noise, and no
+    # coefficient strays beyond 2^-6 of the torus (a quarter of the 1/16 gate decision margin)
+    assert rms[1] <= 2.0 * rms[0] + 2.0**18
+    assert worst[1] < 2**26
+
+
+def test_bootstrap_woks_fp64_bit_exact(gpu_env, rng):
+    ks, dk, keys = gpu_env
+    bits = rng.integers(0, 2, (2, 5)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    x = np.stack([O.gate_combine(P, "NAND", ca[i], cb[i]) for i in range(5)])
+    u, acc = dk.bootstrap_woks(x, prec=0, want_acc=True)
+    for g in range(5):
+        assert np.array_equal(u[g], O.bootstrap_woks(keys, x[g], T().MU)), g
+        assert np.array_equal(u[g], O.extract(P, acc[g])), g
