@@ -38,7 +38,24 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prec", choices=["both", "split32", "fp64"], default="both")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """Reference kernel of known traffic for the FETCH_SIZE/WRITE_SIZE counters: a device-to-device
+    copy moves exactly `mib` MiB in and out per dispatch."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"calibrate_copy_bytes": mib << 20, "dispatches": 3}), flush=True)
 
 
 def log(*a):
@@ -121,6 +138,9 @@ def cpu_baseline(ks, batch_bits, seconds):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)

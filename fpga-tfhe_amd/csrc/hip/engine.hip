@@ -20,6 +20,14 @@ namespace tfhe {
 
 constexpr int kMaxLweN = 1024;   // largest LWE dimension the LDS prologue holds
 
+// Resident wavefronts per SIMD the bootstrap kernels are compiled for (register budget 512/W).
+#ifndef TFHE_BOOT64_WAVES
+#define TFHE_BOOT64_WAVES 1
+#endif
+#ifndef TFHE_BOOT32_WAVES
+#define TFHE_BOOT32_WAVES 2
+#endif
+
 struct BootKArgs {
     BootLaunch L;
     const void* bk;
@@ -97,9 +105,30 @@ __device__ __forceinline__ void boot_epilogue(const BootKArgs& A, int g, int lan
 // Decomposition and rotation are exact integer work; the external product runs the folded FP64
 // transform with the twist folded into the pass-1 twiddles; the result is rounded to nearest and
 // reduced mod 2^32 (the reference truncates, TorusPolynomial_fft.cpp:40).
+//
+// Key rows stream through an LDS buffer with the gfx950 direct global->LDS loads: the row that
+// the next digit needs is requested as soon as the current row has been consumed, so its memory
+// latency hides behind a full forward transform without holding any registers.
 // ---------------------------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
+
+// all lanes request row `row` of the key row set `kr` (2 polynomials of M complex) into kbuf;
+// lane L of the request for (poly, r) lands at element (poly*R1 + r)*64 + L
+template <int M>
+__device__ __forceinline__ void key_row_to_lds(C2<double>* kbuf, const C2<double>* __restrict__ kr, int row,
+                                               int lane) {
+    constexpr int R1 = M / 64;
+    const C2<double>* src = kr + (size_t)row * 2 * M + lane;
+    // the previous row's reads from kbuf must have returned before it is overwritten
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+#pragma unroll
+    for (int pr = 0; pr < 2 * R1; ++pr)
+        __builtin_amdgcn_global_load_lds((const void*)(src + pr * 64), (lds_void*)(kbuf + pr * 64), 16, 0, 0);
+}
+
 template <int N, int L, int BG, bool ROT>
-__device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, int a, const C2<double>* __restrict__ kr,
+__device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, C2<double>* kbuf, int a,
+                                       const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
                                        const LaneTw<double, N / 128>& tw, const C2<double>* wa, int lane,
                                        double* raw) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
@@ -117,10 +146,7 @@ __device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, int a,
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
-            const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M;
-            C2<double> k0[R1], k1[R1];
-#pragma unroll
-            for (int r = 0; r < R1; ++r) { k0[r] = k[r * 64 + lane]; k1[r] = k[M + r * 64 + lane]; }
+            const int row = q * L + lvl;
             C2<double> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
@@ -129,7 +155,11 @@ __device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, int a,
             }
             fft_fwd<R1>(v, tw, s_buf, lane);
 #pragma unroll
-            for (int r = 0; r < R1; ++r) { cmac(o0[r], v[r], k0[r]); cmac(o1[r], v[r], k1[r]); }
+            for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], kbuf[r * 64 + lane]);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], kbuf[(R1 + r) * 64 + lane]);
+            if (row + 1 < 2 * L) key_row_to_lds<M>(kbuf, kr, row + 1, lane);
+            else key_row_to_lds<M>(kbuf, kr_next, 0, lane);
         }
     }
     fft_inv<R1>(o0, tw, s_buf, lane);
@@ -158,10 +188,12 @@ __device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, int a,
 }
 
 template <int N, int L, int BG>
-__global__ void __launch_bounds__(64) k_boot64(BootKArgs A) {
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boot64(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
     __shared__ C2<double> s_buf[M];
+    __shared__ C2<double> s_key[2 * M];
     __shared__ int16_t s_bara[kMaxLweN];
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
@@ -178,15 +210,29 @@ __global__ void __launch_bounds__(64) k_boot64(BootKArgs A) {
 
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-        step64<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, wa, lane, raw);
+        const C2<double>* kr = key + (size_t)A.L.row_set[g] * ROWSET;
+        key_row_to_lds<M>(s_key, kr, 0, lane);
+        step64<N, L, BG, false>(s_acc, s_buf, s_key, 0, kr, kr, tw, wa, lane, raw);
     } else {
         const int n = A.n;
+        if (n > 0) key_row_to_lds<M>(s_key, key, 0, lane);
         for (int i = 0; i < n; ++i) {
             const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
-            if (a == 0) continue;  // tfhe_blindRotate_FFT.cpp:181
-            step64<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, wa, lane, nullptr);
+            // next key row set; the last step re-requests its own (valid) rows, which keeps the
+            // prefetch unconditional
+            const C2<double>* knext = key + (size_t)(i + 1 < n ? i + 1 : i) * ROWSET;
+            if (a == 0) {
+                // zero rotation: the step is skipped (tfhe_blindRotate_FFT.cpp:181); the buffer must
+                // then hold row 0 of the next key instead (rare: one step in 2N)
+                __builtin_amdgcn_s_waitcnt(0);
+                key_row_to_lds<M>(s_key, knext, 0, lane);
+                continue;
+            }
+            step64<N, L, BG, true>(s_acc, s_buf, s_key, a, key + (size_t)i * ROWSET, knext, tw, wa, lane, nullptr);
         }
     }
+    // the last request (a harmless re-read of a valid row) must land before the LDS is released
+    __builtin_amdgcn_s_waitcnt(0);
     boot_epilogue<N>(A, g, lane, s_acc);
 }
 
@@ -219,10 +265,6 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
-            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
-            float4 k0[R1], k1[R1];
-#pragma unroll
-            for (int r = 0; r < R1; ++r) { k0[r] = k[r * 64 + lane]; k1[r] = k[M + r * 64 + lane]; }
             C2<float> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
@@ -231,12 +273,18 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
                 v[aa] = C2<float>{(float)z.x, (float)z.y};
             }
             fft_fwd<R1>(v, tw, s_buf, lane);
+            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
-                cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
-                cmac(l0[r], v[r], C2<float>{k0[r].z, k0[r].w});
-                cmac(h1[r], v[r], C2<float>{k1[r].x, k1[r].y});
-                cmac(l1[r], v[r], C2<float>{k1[r].z, k1[r].w});
+                const float4 k0 = k[r * 64 + lane];
+                cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
+                cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
+            }
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const float4 k1 = k[M + r * 64 + lane];
+                cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
+                cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
             }
         }
     }
@@ -277,7 +325,8 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 }
 
 template <int N, int L, int BG>
-__global__ void __launch_bounds__(64) k_boot32(BootKArgs A) {
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(TFHE_BOOT32_WAVES, TFHE_BOOT32_WAVES))) k_boot32(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
     __shared__ C2<float> s_buf[M];
@@ -312,70 +361,117 @@ __global__ void __launch_bounds__(64) k_boot32(BootKArgs A) {
 
 // ---------------------------------------------------------------------------------------------
 // Keyswitch (lwe-keyswitch-functions.cpp:114-129, :228-241), bit-exact.
-// Grid (NP/64, ceil(count/G)); lane = output coefficient.  Per input index i the lane loads the
-// t*3 candidate rows ks[i][j][1..3][coef] once and applies them to G ciphertexts; the digit of
-// (ciphertext, i, j) is wave-uniform (scalar), read with v_readlane from a coalesced vector load.
+// Workgroup = KS_GATES ciphertexts (one per lane) x a chunk of C output coefficients (registers).
+// For each input index i the workgroup stages the t*3 candidate rows ks[i][j][1..3][chunk] in LDS
+// (row 0 is a zero row standing for digit 0, which the reference skips), and every lane subtracts
+// row[j][digit] for its own digit: one LDS read and one subtract per (ciphertext, i, j, coef), with
+// each keyswitch-key row read from memory once per workgroup.  The extracted samples are staged
+// through LDS in blocks of KS_IBLK indices so their reads stay coalesced.
 // ---------------------------------------------------------------------------------------------
-template <int TKS, int G>
-__global__ void __launch_bounds__(64) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
-                                                  int32_t add_b, const int32_t* __restrict__ ksk,
-                                                  int32_t* __restrict__ out, int count, int kN, int n, int np) {
-    const int lane = threadIdx.x;
-    const int coef = blockIdx.x * 64 + lane;
-    const int g0 = blockIdx.y * G;
-    const int gcount = min(G, count - g0);
+constexpr int KS_GATES = 256;
+constexpr int KS_IBLK = 16;
+
+template <int TKS, int KN, int NP, int C>
+__global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
+                                                        int32_t add_b, const int32_t* __restrict__ ksk,
+                                                        int32_t* __restrict__ out, int count, int n) {
+    static_assert(KS_GATES % C == 0 && NP % C == 0 && KN % KS_IBLK == 0, "keyswitch tiling");
+    __shared__ uint4 s_k[2][TKS][4][C / 4];
+    __shared__ uint32_t s_u[KS_IBLK][KS_GATES];
+    constexpr size_t UST = (size_t)KN + 1;
+    constexpr int KV = TKS * 3 * C;                  // staged values per input index
+    constexpr int KPT = (KV + KS_GATES - 1) / KS_GATES;
+    constexpr int ROWS_PER_T = KS_GATES / C;         // (j, h) rows covered by one pass of the block
+    const int tid = threadIdx.x;
+    const int g0 = blockIdx.y * KS_GATES;
+    const int g = g0 + tid;
+    const int c0 = blockIdx.x * C;
     const uint32_t prec_off = 1u << (32 - (1 + 2 * TKS));
-    const size_t ust = (size_t)kN + 1;
-    uint32_t acc[G];
+
+    uint32_t acc[C];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g] = 0;
-    if (coef == n) {
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-            if (g < gcount) {
-                uint32_t b = (uint32_t)u1[(g0 + g) * ust + kN] + (uint32_t)add_b;
-                if (u2) b += (uint32_t)u2[(g0 + g) * ust + kN];
-                acc[g] = b;
-            }
+    for (int c = 0; c < C; ++c) acc[c] = 0;
+    for (int idx = tid; idx < 2 * TKS * (C / 4); idx += KS_GATES) {
+        const int b = idx / (TKS * (C / 4)), r = idx % (TKS * (C / 4));
+        s_k[b][r / (C / 4)][0][r % (C / 4)] = make_uint4(0, 0, 0, 0);
     }
-    for (int i0 = 0; i0 < kN; i0 += 64) {
-        uint32_t av[G];
+    // staged value v = tid + t * KS_GATES covers row jh = v / C = tid / C + t * ROWS_PER_T and
+    // column cc = tid % C of the chunk
+    const int cc = tid % C, jh0 = tid / C;
+    const int32_t* kbase = ksk + (size_t)jh0 * NP + c0 + cc;
+    uint32_t* sk_flat = reinterpret_cast<uint32_t*>(&s_k[0][0][0][0]);
+    uint32_t pre[KPT];
+    auto load_rows = [&](int i) {
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            uint32_t v = 0;
-            if (g < gcount) {
-                v = (uint32_t)u1[(g0 + g) * ust + i0 + lane];
-                if (u2) v += (uint32_t)u2[(g0 + g) * ust + i0 + lane];
-            }
-            av[g] = v + prec_off;
+        for (int t = 0; t < KPT; ++t) {
+            const int jh = jh0 + t * ROWS_PER_T;
+            pre[t] = (jh < TKS * 3) ? (uint32_t)kbase[((size_t)i * TKS * 3 + t * ROWS_PER_T) * NP] : 0u;
         }
-        for (int ii = 0; ii < 64; ++ii) {
+    };
+    auto store_rows = [&](int b) {
+#pragma unroll
+        for (int t = 0; t < KPT; ++t) {
+            const int jh = jh0 + t * ROWS_PER_T;
+            if (jh < TKS * 3) sk_flat[((b * TKS + jh / 3) * 4 + jh % 3 + 1) * C + cc] = pre[t];
+        }
+    };
+    load_rows(0);
+    store_rows(0);
+    // u staging: thread reads row gg = tid / KS_IBLK + t * (KS_GATES / KS_IBLK), column ii = tid % KS_IBLK
+    constexpr int GSTEP = KS_GATES / KS_IBLK;
+    const int ii0 = tid % KS_IBLK, gg0 = tid / KS_IBLK;
+    const int32_t* u1b = u1 + (size_t)(g0 + gg0) * UST + ii0;
+    const int32_t* u2b = u2 ? u2 + (size_t)(g0 + gg0) * UST + ii0 : nullptr;
+    for (int i0 = 0; i0 < KN; i0 += KS_IBLK) {
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < KS_IBLK; ++t) {
+            const int gg = gg0 + t * GSTEP;
+            uint32_t x = 0;
+            if (g0 + gg < count) {
+                x = (uint32_t)u1b[(size_t)t * GSTEP * UST + i0];
+                if (u2b) x += (uint32_t)u2b[(size_t)t * GSTEP * UST + i0];
+            }
+            s_u[ii0][gg] = x + prec_off;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int ii = 0; ii < KS_IBLK; ++ii) {
             const int i = i0 + ii;
-            uint32_t kv[TKS][3];
-            const int32_t* kp = ksk + (size_t)i * TKS * 3 * np + coef;
+            const int b = i & 1;
+            if (i + 1 < KN) load_rows(i + 1);
+            const uint32_t aibar = s_u[ii][tid];
+            // not unrolled: an unrolled j loop lets the compiler re-associate the eight row
+            // subtractions into one sum and keep all eight rows live (four times the registers)
+#pragma unroll 1
+            for (int j = 0; j < TKS; ++j) {
+                const uint32_t d = (aibar >> (30 - 2 * j)) & 3u;
+                const uint4* row = &s_k[b][j][d][0];
 #pragma unroll
-            for (int j = 0; j < TKS; ++j)
-#pragma unroll
-                for (int h = 0; h < 3; ++h) kv[j][h] = (uint32_t)kp[(j * 3 + h) * np];
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                if (g < gcount) {
-                    const uint32_t ai = (uint32_t)__builtin_amdgcn_readlane((int)av[g], ii);
-#pragma unroll
-                    for (int j = 0; j < TKS; ++j) {
-                        const uint32_t d = (ai >> (30 - 2 * j)) & 3u;
-                        if (d == 1) acc[g] -= kv[j][0];
-                        else if (d == 2) acc[g] -= kv[j][1];
-                        else if (d == 3) acc[g] -= kv[j][2];
-                    }
+                for (int c4 = 0; c4 < C / 4; ++c4) {
+                    const uint4 kv = row[c4];
+                    acc[4 * c4 + 0] -= kv.x;
+                    acc[4 * c4 + 1] -= kv.y;
+                    acc[4 * c4 + 2] -= kv.z;
+                    acc[4 * c4 + 3] -= kv.w;
                 }
             }
+            if (i + 1 < KN) store_rows(b ^ 1);
+            __syncthreads();
         }
     }
-    if (coef <= n) {
+    if (g < count) {
+        if (n >= c0 && n < c0 + C) {
+            uint32_t bterm = (uint32_t)u1[(size_t)g * UST + KN] + (uint32_t)add_b;
+            if (u2) bterm += (uint32_t)u2[(size_t)g * UST + KN];
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-            if (g < gcount) out[(size_t)(g0 + g) * (n + 1) + coef] = (int32_t)acc[g];
+            for (int c = 0; c < C; ++c)
+                if (c0 + c == n) acc[c] += bterm;
+        }
+        int32_t* o = out + (size_t)g * (n + 1);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (c0 + c <= n) o[c0 + c] = (int32_t)acc[c];
     }
 }
 
@@ -684,22 +780,25 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
     return hipErrorInvalidValue;
 }
 
-template <int TKS>
+template <int TKS, int KN, int NP>
 static hipError_t launch_ks_t(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
                               int32_t add_b, int count, hipStream_t st) {
-    constexpr int G = 32;
-    const int kN = key.p.N;
-    dim3 grid(key.ksk_np / 64, (count + G - 1) / G);
-    hipLaunchKernelGGL((k_keyswitch<TKS, G>), grid, dim3(64), 0, st, u1, u2, add_b, key.ksk, out, count, kN,
-                       key.p.n, key.ksk_np);
+    constexpr int C = 32;
+    if (key.ksk_np != NP || key.p.N != KN) return hipErrorInvalidValue;
+    dim3 grid(NP / C, (count + KS_GATES - 1) / KS_GATES);
+    hipLaunchKernelGGL((k_keyswitch<TKS, KN, NP, C>), grid, dim3(KS_GATES), 0, st, u1, u2, add_b, key.ksk, out, count,
+                       key.p.n);
     return hipGetLastError();
 }
 
 hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2, int32_t add_b,
                             int count, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    if (key.p.ks_t == 8) return launch_ks_t<8>(key, out, u1, u2, add_b, count, st);
-    if (key.p.ks_t == 10) return launch_ks_t<10>(key, out, u1, u2, add_b, count, st);
+    // the two supported parameter sets: (t, N, padded n+1) = (8, 1024, 640) and (10, 2048, 768)
+    if (key.p.ks_t == 8 && key.p.N == 1024 && key.ksk_np == 640)
+        return launch_ks_t<8, 1024, 640>(key, out, u1, u2, add_b, count, st);
+    if (key.p.ks_t == 10 && key.p.N == 2048 && key.ksk_np == 768)
+        return launch_ks_t<10, 2048, 768>(key, out, u1, u2, add_b, count, st);
     return hipErrorInvalidValue;
 }
 

@@ -148,7 +148,15 @@ __device__ __forceinline__ void load_lane_tw(LaneTw<T, R1>& tw, const C2<T>* __r
     for (int r = 0; r < 8; ++r) tw.t2[r] = g_t2[lane * 8 + r];
 }
 
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// Exchange fence for the transforms.  Every transform is owned by one wavefront and its
+// workgroup is that wavefront alone; LDS operations of a wavefront execute in program order, so
+// ordering the compiler (a wavefront-scope fence emits no instruction) is all that is needed --
+// no s_barrier and no lgkmcnt(0) drain between the exchange write and read.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // exchange-2 address: element (k1 = k1lo + 8 s, k2, c); conflict-free for writes by lane
 // (c, k1lo) and reads by lane (k1lo, k2) thanks to the c ^ k1lo swizzle

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libtfhe_mi355x.so")
+# TFHE_MI355X_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("TFHE_MI355X_LIB") or os.path.join(_HERE, "lib", "libtfhe_mi355x.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "tfhe_mi355x.h")
 
 PREC_FP64, PREC_SPLIT32 = 0, 1
