@@ -111,6 +111,23 @@ def audit(T, dk, ks, prec, batch, total, rng):
     return done, fails
 
 
+def precision_report(T, dk, rng, count=64):
+    """Per-coefficient error of the split-FP32 external product.  The reference value is the FP64
+    path's result, which is the exact int64 negacyclic product whenever its own pre-rounding
+    values lie within 1/2 of an integer (checked here and reported as fp64_max_dist_to_integer;
+    the parity tests hold it bit-exact against the int64 oracle)."""
+    acc = rng.integers(-2**31, 2**31, (count, 2, N), dtype=np.int64).astype(np.int32)
+    rows = rng.integers(0, n, count).astype(np.int32)
+    r64, raw64 = dk.extern_product(acc, rows, prec=T.PREC_FP64, want_raw=True)
+    r32 = dk.extern_product(acc, rows, prec=T.PREC_SPLIT32)
+    dist = float(np.abs(raw64 - np.rint(raw64)).max())
+    err = np.abs(((r32.astype(np.int64) - r64.astype(np.int64) + 2**31) % 2**32) - 2**31).ravel()
+    hist = np.bincount(np.ceil(np.log2(err.astype(np.float64) + 1)).astype(int), minlength=1)
+    return {"samples": int(err.size), "max": int(err.max()), "rms": round(float(np.sqrt((err.astype(float) ** 2).mean())), 1),
+            "unit": "2^-32 torus", "log2_histogram": {f"<=2^{i}": int(c) for i, c in enumerate(hist) if c},
+            "fp64_max_dist_to_integer": dist}
+
+
 def cpu_baseline(ks, batch_bits, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -172,12 +189,15 @@ def main():
         log(f"[bench] {name}: {r}")
 
     head = "split32" if "split32" in res else "fp64"
-    aud = None
+    aud = {}
     if args.audit_gates > 0:
-        t = time.perf_counter()
-        done, fails = audit(T, dk, ks, precs[head], B, args.audit_gates, np.random.default_rng(2))
-        aud = {"gates": done, "failures": fails, "precision": head, "chained": True}
-        log(f"[bench] audit {aud} in {time.perf_counter() - t:.1f} s")
+        for name, prec in precs.items():
+            t = time.perf_counter()
+            done, fails = audit(T, dk, ks, prec, B, args.audit_gates, np.random.default_rng(2))
+            aud[name] = {"gates": done, "failures": fails, "chained": True}
+            log(f"[bench] audit {name} {aud[name]} in {time.perf_counter() - t:.1f} s")
+    prec_rep = precision_report(T, dk, np.random.default_rng(3)) if len(precs) == 2 else None
+    log(f"[bench] split32 external-product error {prec_rep}")
 
     cpu = None
     if not args.no_cpu:
@@ -185,6 +205,8 @@ def main():
         cpu = cpu_baseline(ks, bits, args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu} in {time.perf_counter() - t:.1f} s")
 
+    vp64, vp32 = T.measure_valu_peak()
+    log(f"[bench] measured dense-FMA VALU peak: fp64 {vp64:.1f} TFLOP/s, fp32 {vp32:.1f} TFLOP/s")
     hr = res[head]
     achieved = FLOP_PER_GATE * B / (hr["boot_ms_avg"] * 1e-3) / 1e12
     traffic = None
@@ -213,10 +235,12 @@ def main():
                        "correct": v["correct"]} for kk, v in res.items()},
         "split32_over_fp64": (round(res["split32"]["gates_per_s"] / res["fp64"]["gates_per_s"], 3)
                               if len(res) == 2 else None),
-        "audit": aud,
+        "audit": aud or None,
+        "split32_extern_product_error": prec_rep,
         "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),
                      "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
-                     "traffic": traffic, "kernel": f"bootstrap_{head}", "flop_per_gate": FLOP_PER_GATE},
+                     "traffic": traffic, "kernel": f"bootstrap_{head}", "flop_per_gate": FLOP_PER_GATE,
+                     "measured_valu_peak": {"fp64": round(vp64, 1), "fp32_unpacked": round(vp32, 1)}},
         "cpu_baseline": cpu,
         "device": dev,
     }

@@ -74,6 +74,7 @@ hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* di
                                    const int32_t* bara, int count, hipStream_t st);
 hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse, double* out, const void* in,
                            int count, hipStream_t st);
+hipError_t launch_valu_probe(int fp64, void* out, int blocks, int iters, hipStream_t st);
 hipError_t launch_decrypt(int32_t* bits, const int32_t* samples, const int32_t* key, int n, int count,
                           hipStream_t st);
 

@@ -678,6 +678,21 @@ __global__ void __launch_bounds__(64) k_decrypt(int32_t* __restrict__ bits, cons
     if (lane == 0) bits[g] = ((int32_t)((uint32_t)x[n] - acc) > 0) ? 1 : 0;
 }
 
+// Dense FMA chains (8 independent per lane) to measure the attainable VALU rate of one precision.
+template <typename T>
+__global__ void __launch_bounds__(256) k_valu_probe(T* __restrict__ out, T a, T b, int iters) {
+    T v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = (T)(threadIdx.x + c);
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = v[c] * a + b;
+    T s = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += v[c];
+    out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 // =============================================================================================
 // host side: twiddle tables and launchers
 // =============================================================================================
@@ -860,6 +875,14 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<double>*)tw.twist64);
         else return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(int fp64, void* out, int blocks, int iters, hipStream_t st) {
+    if (fp64)
+        hipLaunchKernelGGL((k_valu_probe<double>), dim3(blocks), dim3(256), 0, st, (double*)out, 1.0000001, 1e-9, iters);
+    else
+        hipLaunchKernelGGL((k_valu_probe<float>), dim3(blocks), dim3(256), 0, st, (float*)out, 1.0000001f, 1e-9f, iters);
     return hipGetLastError();
 }
 

@@ -851,6 +851,39 @@ extern "C" int tfhe_gpu_decrypt_batch_dev(int32_t* bits, const int32_t* d_sample
     return d2h_sync(bits, db, (size_t)count * 4);
 }
 
+// ---- attainable VALU rate (roofline denominator check) ----
+extern "C" int tfhe_gpu_measure_valu_peak(double* fp64_tflops, double* fp32_tflops) {
+    Ctx& c = ctx();
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    int rc = ensure_init();
+    if (rc) return rc;
+    int cus = 0;
+    HIPR(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    const int blocks = cus * 16, iters = 4096;
+    void* buf;
+    if ((rc = scratch().get(6, (size_t)blocks * 256 * 8, &buf))) return rc;
+    double* outs[2] = {fp32_tflops, fp64_tflops};
+    for (int fp64 = 0; fp64 < 2; ++fp64) {
+        hipEvent_t e0, e1;
+        HIPR(hipEventCreate(&e0));
+        HIPR(hipEventCreate(&e1));
+        hipError_t e = launch_valu_probe(fp64, buf, blocks, iters, c.stream);  // warm-up
+        if (e != hipSuccess) return hip_fail("valu probe", e);
+        HIPR(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 3; ++r)
+            if ((e = launch_valu_probe(fp64, buf, blocks, iters, c.stream)) != hipSuccess) return hip_fail("valu probe", e);
+        HIPR(hipEventRecord(e1, c.stream));
+        HIPR(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPR(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double flop = 3.0 * blocks * 256.0 * iters * 8 * 2;
+        if (outs[fp64]) *outs[fp64] = flop / (ms * 1e-3) / 1e12;
+    }
+    return TFHE_OK;
+}
+
 // ---- profiling ----
 extern "C" int tfhe_gpu_profile_enable(int32_t on) {
     Ctx& c = ctx();

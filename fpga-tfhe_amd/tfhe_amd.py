@@ -105,6 +105,7 @@ def lib():
         "tfhe_gpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
         "tfhe_gpu_memcpy_d2h": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
         "tfhe_gpu_decrypt_batch_dev": (ctypes.c_int, [_i32p, _vp, _i32p, i32, i32]),
+        "tfhe_gpu_measure_valu_peak": (ctypes.c_int, [P(ctypes.c_double), P(ctypes.c_double)]),
         "tfhe_gpu_profile_enable": (ctypes.c_int, [i32]),
         "tfhe_gpu_profile_reset": (ctypes.c_int, []),
         "tfhe_gpu_profile_read": (ctypes.c_int, [ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_int64)]),
@@ -364,3 +365,10 @@ def profile_read(kernel):
     n = ctypes.c_int64()
     _ck(lib().tfhe_gpu_profile_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "profile_read")
     return ms.value, n.value
+
+
+def measure_valu_peak():
+    """Attainable dense-FMA VALU TFLOP/s (fp64, fp32) on this device."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _ck(lib().tfhe_gpu_measure_valu_peak(ctypes.byref(a), ctypes.byref(b)), "measure_valu_peak")
+    return a.value, b.value

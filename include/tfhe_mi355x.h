@@ -363,6 +363,9 @@ int tfhe_gpu_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int tfhe_gpu_decrypt_batch_dev(int32_t* bits, const int32_t* d_samples, const int32_t* lwe_key,
                                int32_t n, int32_t count);
 
+/* attainable dense-FMA VALU rate of each precision on this device (roofline denominator check) */
+int tfhe_gpu_measure_valu_peak(double* fp64_tflops, double* fp32_tflops);
+
 /* kernel timing on the library stream (hipEvent pairs around each launch) */
 int tfhe_gpu_profile_enable(int32_t on);
 int tfhe_gpu_profile_reset(void);

@@ -166,6 +166,10 @@ def t32tod(x):
     return lib().orc_t32tod(int(x))
 
 
+def gaussian32(message, sigma):
+    return lib().orc_gaussian32(int(message), float(sigma))
+
+
 def approx_phase(phase, msize):
     return lib().orc_approx_phase(int(phase), int(msize))
 
