@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boot64(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
-    __shared__ C2<double> s_buf[M];
+    __shared__ C2<double> s_buf[xbuf_elems<M>()];
     __shared__ C2<double> s_key[2 * M];
     __shared__ int16_t s_bara[kMaxLweN];
     const int lane = threadIdx.x;
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(TFHE_BOOT32_WAVES, TFHE_BOOT32_WAVES))) k_boot32(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
-    __shared__ C2<float> s_buf[M];
+    __shared__ C2<float> s_buf[xbuf_elems<M>()];
     __shared__ int16_t s_bara[kMaxLweN];
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
@@ -376,7 +376,9 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
                                                         int32_t add_b, const int32_t* __restrict__ ksk,
                                                         int32_t* __restrict__ out, int count, int n) {
     static_assert(KS_GATES % C == 0 && NP % C == 0 && KN % KS_IBLK == 0, "keyswitch tiling");
-    __shared__ uint4 s_k[2][TKS][4][C / 4];
+    // candidate rows, each padded by one uint4 so the 4 rows of a (j) start in distinct banks
+    constexpr int KROW = C / 4 + 1;
+    __shared__ uint4 s_k[2][TKS][4][KROW];
     __shared__ uint32_t s_u[KS_IBLK][KS_GATES];
     constexpr size_t UST = (size_t)KN + 1;
     constexpr int KV = TKS * 3 * C;                  // staged values per input index
@@ -412,7 +414,7 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
 #pragma unroll
         for (int t = 0; t < KPT; ++t) {
             const int jh = jh0 + t * ROWS_PER_T;
-            if (jh < TKS * 3) sk_flat[((b * TKS + jh / 3) * 4 + jh % 3 + 1) * C + cc] = pre[t];
+            if (jh < TKS * 3) sk_flat[((b * TKS + jh / 3) * 4 + jh % 3 + 1) * (KROW * 4) + cc] = pre[t];
         }
     };
     load_rows(0);
@@ -486,7 +488,7 @@ __global__ void __launch_bounds__(64) k_keyprep(const int32_t* __restrict__ bk, 
                                                 const C2<double>* __restrict__ t1, const C2<double>* __restrict__ t2,
                                                 const C2<double>* __restrict__ wa64) {
     constexpr int M = N / 2, R1 = M / 64;
-    __shared__ C2<double> s_buf[M];
+    __shared__ C2<double> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
     const size_t poly = blockIdx.x;
     const int32_t* p = bk + poly * N;
@@ -577,7 +579,7 @@ __global__ void __launch_bounds__(64) k_fft_test64(double* __restrict__ out, con
                                                    const C2<double>* __restrict__ t1, const C2<double>* __restrict__ t2,
                                                    const C2<double>* __restrict__ wa64) {
     constexpr int M = N / 2, R1 = M / 64;
-    __shared__ C2<double> s_buf[M];
+    __shared__ C2<double> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
     const size_t g = blockIdx.x;
     LaneTw<double, R1> tw;
@@ -623,7 +625,7 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
                                                    const C2<float>* __restrict__ t1, const C2<float>* __restrict__ t2,
                                                    const C2<double>* __restrict__ twist64) {
     constexpr int M = N / 2, R1 = M / 64;
-    __shared__ C2<float> s_buf[M];
+    __shared__ C2<float> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
     const size_t g = blockIdx.x;
     LaneTw<float, R1> tw;

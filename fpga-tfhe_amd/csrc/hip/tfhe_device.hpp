@@ -130,8 +130,8 @@ __device__ __forceinline__ void dftR(C2<T>* v) {
 }
 
 // Per-lane twiddles of one transform size, loaded once per kernel.
-//   t1[k1] = pass-1 twiddle for lane (b,c): W_M^{(8b+c) k1}, times w^{8b+c} when the twist is
-//            folded in (FP64 path)
+//   t1[k1] = pass-1 twiddle for lane (b,c): W_M^{(8b+c) k1} times the lane part w^{8b+c} of the
+//            twist, so the forward twist reduces to the register constants w^{64a}
 //   t2[k2] = pass-2 twiddle for lane (c,k1): W_64^{c k2}
 template <typename T, int R1>
 struct LaneTw {
@@ -158,11 +158,24 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// exchange-2 address: element (k1 = k1lo + 8 s, k2, c); conflict-free for writes by lane
-// (c, k1lo) and reads by lane (k1lo, k2) thanks to the c ^ k1lo swizzle
+// LDS exchange addresses.  Each access of a wavefront (one register index, 64 lanes) should touch
+// distinct banks within every group of lanes the LDS serves per cycle (16 or 32 lanes for 8-byte
+// elements, 8 or 16 for 16-byte elements, with 32 or 64 banks), and the address must stay a
+// per-lane base plus a per-register constant so it folds into the instruction offset.
+//
+// exchange 1: element (k1, b, c) -- written by lane 8b+c (register k1), read by lane c+8(k1 mod 8)
+// (register b).  Rows of 64 are padded to 72, so the 8-lane runs of consecutive k1 start 8 slots
+// apart modulo any bank width.
+constexpr int X1_ROW = 72;
+__device__ __forceinline__ int x1_addr(int k1, int b, int c) { return k1 * X1_ROW + 8 * b + c; }
+// exchange 2: element (k1 = k1lo + 8 s, k2, c) -- written by lane c+8k1lo (register k2), read by
+// lane k1lo+8k2 (register c); the c ^ k1lo swizzle spreads the reads of one register over banks
 __device__ __forceinline__ int x2_addr(int s, int k2, int k1lo, int c) {
     return (s * 8 + k2) * 64 + 8 * k1lo + (c ^ k1lo);
 }
+// LDS elements an exchange buffer needs for an M-point transform
+template <int M>
+__host__ __device__ constexpr int xbuf_elems() { return (M / 64) * X1_ROW > M ? (M / 64) * X1_ROW : M; }
 
 // Forward transform of one wavefront-owned polynomial.  In: v[a] = z_{64a + lane} (already
 // multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
@@ -173,14 +186,17 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>
     dftR<R1, +1>(v);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
+    {
+        const int b1 = lane >> 3, c1 = lane & 7;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[k1 * 64 + lane] = v[k1];
+        for (int k1 = 0; k1 < R1; ++k1) buf[x1_addr(k1, b1, c1)] = v[k1];
+    }
     wave_sync();
     const int c = lane & 7, k1lo = lane >> 3;
 #pragma unroll
     for (int s = 0; s < G; ++s)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) v[s * 8 + b] = buf[(k1lo + 8 * s) * 64 + 8 * b + c];
+        for (int b = 0; b < 8; ++b) v[s * 8 + b] = buf[x1_addr(k1lo + 8 * s, b, c)];
     wave_sync();
 #pragma unroll
     for (int s = 0; s < G; ++s) {
@@ -230,11 +246,14 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>
         for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
         dft8<-1>(v + s * 8);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) buf[(k1lo + 8 * s) * 64 + 8 * b + c] = v[s * 8 + b];
+        for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo + 8 * s, b, c)] = v[s * 8 + b];
     }
     wave_sync();
+    {
+        const int b1 = lane >> 3, c1 = lane & 7;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[k1 * 64 + lane];
+        for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[x1_addr(k1, b1, c1)];
+    }
     wave_sync();
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
