@@ -237,6 +237,153 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
 }
 
 // ---------------------------------------------------------------------------------------------
+// FP64 path, key-sharing form.  A workgroup of B64W_GATES wavefronts runs that many bootstraps in
+// lockstep over the key: every key row is brought into LDS once per workgroup, each wavefront
+// loading its share with direct global->LDS loads, instead of once per bootstrap.  Per-bootstrap
+// LDS (accumulator, half-width exchange buffer, bara) is small enough for two wavefronts per SIMD,
+// which is what hides the transform's LDS latency.  Rows rotate through three buffers: row t+2 is
+// requested right after the barrier of row t, when every wavefront has finished with row t-1, and
+// has two transforms' time to land.
+// ---------------------------------------------------------------------------------------------
+constexpr int B64W_GATES = 8;
+constexpr int B64W_MAXN = 640;   // LWE dimension the per-bootstrap bara array holds
+
+template <int M, int W>
+__device__ __forceinline__ void key_row_share(C2<double>* kbuf, const C2<double>* __restrict__ src, int wave,
+                                              int lane) {
+    constexpr int R1 = M / 64, PER = 2 * R1 / W;
+    static_assert((2 * R1) % W == 0, "a key row splits evenly over the wavefronts");
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int pr = wave * PER + p;
+        __builtin_amdgcn_global_load_lds((const void*)(src + pr * 64 + lane), (lds_void*)(kbuf + pr * 64), 16, 0, 0);
+    }
+}
+
+// Workgroup execution barrier for the shared key rows.  The rows arrive through direct
+// global->LDS loads, whose completion each wavefront establishes with its own vmcnt wait before
+// the barrier; LDS is coherent across the CU, so no cache maintenance is needed.  The
+// wavefront-scope fences only stop the compiler from moving LDS accesses across the barrier (a
+// workgroup-scope fence would also drain every outstanding load, including the prefetch of the
+// row after next).
+__device__ __forceinline__ void rows_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// s_waitcnt with only vmcnt set (gfx9 encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
+template <int VM>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(VM >= 0 && VM < 16, "vmcnt");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | VM);
+}
+
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64 * B64W_GATES)
+__attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
+    static_assert(L == 3, "buffer rotation assumes 3 levels: row t of step i lands in buffer t mod 3 = lvl");
+    constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1, KPL = 2 * L, W = B64W_GATES;
+    constexpr int PER = 2 * R1 / W;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    constexpr size_t ROW = (size_t)2 * M, ROWSET = (size_t)KPL * ROW;
+    __shared__ int32_t s_acc_all[W][2 * N];
+    __shared__ double s_x_all[W][xbuf_elems<M>()];
+    __shared__ int16_t s_bara_all[W][B64W_MAXN];
+    __shared__ C2<double> s_k0[2 * M];
+    __shared__ C2<double> s_k1[2 * M];
+    __shared__ C2<double> s_k2[2 * M];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = blockIdx.x * W + wave;
+    const bool active = g < A.L.count;
+    int32_t* s_acc = s_acc_all[wave];
+    double* s_x = s_x_all[wave];
+    int16_t* s_bara = s_bara_all[wave];
+    if (active) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    wave_sync();
+
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, (const C2<double>*)A.t1, (const C2<double>*)A.t2, lane);
+    C2<double> wa[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) wa[r] = ((const C2<double>*)A.wa64)[r];
+    const C2<double>* key = (const C2<double>*)A.bk;
+    const int n = A.n;
+    C2<double>* const kb[3] = {s_k0, s_k1, s_k2};
+
+    key_row_share<M, W>(s_k0, key, wave, lane);
+    key_row_share<M, W>(s_k1, key + ROW, wave, lane);
+    for (int i = 0; i < n; ++i) {
+        const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
+        // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); an idle wavefront still
+        // takes part in the barriers and in loading the shared rows
+        const bool work = a != 0;
+        C2<double> o0[R1], o1[R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            uint32_t u[NC];
+            if (work) {
+#pragma unroll
+                for (int r = 0; r < NC; ++r) u[r] = rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF;
+            }
+#pragma unroll
+            for (int lvl = 0; lvl < L; ++lvl) {
+                const int row = q * L + lvl;
+                C2<double> v[R1];
+                if (work) {
+#pragma unroll
+                    for (int aa = 0; aa < R1; ++aa) {
+                        v[aa] = C2<double>{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
+                        if (aa) v[aa] = cmul(v[aa], wa[aa]);
+                    }
+                    fft_fwd<R1, double, true>(v, tw, s_x, lane);
+                }
+                // this wavefront's share of row t has landed (only row t+1's loads may remain),
+                // and after the barrier every share has, and row t-1 is free
+                wait_vmcnt<PER>();
+                rows_barrier();
+                {
+                    const int r2 = row + 2;
+                    const C2<double>* src = (r2 < KPL) ? key + (size_t)i * ROWSET + (size_t)r2 * ROW
+                                                       : key + (size_t)(i + 1 < n ? i + 1 : i) * ROWSET +
+                                                             (size_t)(r2 - KPL) * ROW;
+                    key_row_share<M, W>(kb[(lvl + 2) % 3], src, wave, lane);
+                }
+                if (work) {
+                    const C2<double>* k = kb[lvl];
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64 + lane]);
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], k[(R1 + r) * 64 + lane]);
+                }
+            }
+        }
+        if (work) {
+            fft_inv<R1, double, true>(o0, tw, s_x, lane);
+            fft_inv<R1, double, true>(o1, tw, s_x, lane);
+            const double sc = 1.0 / (double)M;
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const int j = lane + 64 * aa;
+                C2<double> z0 = aa ? cmulc(o0[aa], wa[aa]) : o0[aa];
+                C2<double> z1 = aa ? cmulc(o1[aa], wa[aa]) : o1[aa];
+                s_acc[j] = (int32_t)((uint32_t)s_acc[j] + round_mod32(z0.x * sc));
+                s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + round_mod32(z0.y * sc));
+                s_acc[N + j] = (int32_t)((uint32_t)s_acc[N + j] + round_mod32(z1.x * sc));
+                s_acc[N + j + M] = (int32_t)((uint32_t)s_acc[N + j + M] + round_mod32(z1.y * sc));
+            }
+            wave_sync();
+        }
+    }
+    // the last two requests (re-reads of valid rows) must land before the LDS is released
+    __builtin_amdgcn_s_waitcnt(0);
+    if (active) boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Split-FP32 path.  Each key coefficient K = Khi * 2^s + Klo with Klo in [-2^(s-1), 2^(s-1));
 // both halves are stored as FP32 transforms.  Digits are twisted in FP64 and rounded to FP32,
 // transformed and multiplied in FP32, and the hi and lo products are inverse-transformed
@@ -765,6 +912,10 @@ bool params_supported(const DevParams& p) {
     return ring && p.n > 0 && p.n <= kMaxLweN && p.ks_basebit == 2 && (p.ks_t == 8 || p.ks_t == 10);
 }
 
+// FP64 bootstrap form: workgroups sharing key rows (default) or one bootstrap per workgroup
+static bool g_boot64_shared = true;
+void set_boot64_shared(bool on) { g_boot64_shared = on; }
+
 template <int N, int L, int BG>
 static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
                                 hipStream_t st) {
@@ -778,6 +929,13 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.bk = key.bk64;
         A.t1 = tw.t1_fold64;
         A.t2 = tw.t2_64;
+        if constexpr (N == 1024) {
+            if (Lc.mode != 2 && key.p.n <= B64W_MAXN && g_boot64_shared) {
+                const int blocks = (Lc.count + B64W_GATES - 1) / B64W_GATES;
+                hipLaunchKernelGGL((k_boot64w<N, L, BG>), dim3(blocks), dim3(64 * B64W_GATES), 0, st, A);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((k_boot64<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
     } else {
         A.bk = key.bk32;

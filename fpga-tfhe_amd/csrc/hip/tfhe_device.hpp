@@ -177,84 +177,145 @@ __device__ __forceinline__ int x2_addr(int s, int k2, int k1lo, int c) {
 template <int M>
 __host__ __device__ constexpr int xbuf_elems() { return (M / 64) * X1_ROW > M ? (M / 64) * X1_ROW : M; }
 
+// One LDS exchange: register r of every lane is stored at address wr(r), then register r is
+// loaded from address rd(r).  HALF exchanges the real and imaginary parts one after the other
+// through a buffer of scalars, halving the LDS footprint for the same bytes moved.
+template <int R, bool HALF, typename T, typename WR, typename RD>
+__device__ __forceinline__ void exchange(C2<T>* v, void* buf, WR wr, RD rd) {
+    if constexpr (!HALF) {
+        C2<T>* b = (C2<T>*)buf;
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[wr(r)] = v[r];
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = b[rd(r)];
+        wave_sync();
+    } else {
+        T* b = (T*)buf;
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[wr(r)] = v[r].x;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r].x = b[rd(r)];
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[wr(r)] = v[r].y;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r].y = b[rd(r)];
+        wave_sync();
+    }
+}
+
 // Forward transform of one wavefront-owned polynomial.  In: v[a] = z_{64a + lane} (already
 // multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
-// lane order.  buf: M elements of LDS.
-template <int R1, typename T>
-__device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>* buf, int lane) {
+// lane order.  buf: xbuf_elems<M>() elements of C2<T> (of T when HALF) private to the wavefront.
+template <int R1, typename T, bool HALF = false>
+__device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
     dftR<R1, +1>(v);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
-    {
-        const int b1 = lane >> 3, c1 = lane & 7;
-#pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) buf[x1_addr(k1, b1, c1)] = v[k1];
-    }
-    wave_sync();
     const int c = lane & 7, k1lo = lane >> 3;
+    if constexpr (!HALF) {
+        C2<T>* buf = (C2<T>*)vbuf;
+        {
+            const int b1 = lane >> 3, c1 = lane & 7;
 #pragma unroll
-    for (int s = 0; s < G; ++s)
-#pragma unroll
-        for (int b = 0; b < 8; ++b) v[s * 8 + b] = buf[x1_addr(k1lo + 8 * s, b, c)];
-    wave_sync();
-#pragma unroll
-    for (int s = 0; s < G; ++s) {
-        dft8<+1>(v + s * 8);
-#pragma unroll
-        for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(s, k2, k1lo, c)] = v[s * 8 + k2];
-    }
-    wave_sync();
-    {
-        const int k1lo3 = lane & 7, k2 = lane >> 3;
+            for (int k1 = 0; k1 < R1; ++k1) buf[x1_addr(k1, b1, c1)] = v[k1];
+        }
+        wave_sync();
 #pragma unroll
         for (int s = 0; s < G; ++s)
 #pragma unroll
-            for (int cc = 0; cc < 8; ++cc) v[s * 8 + cc] = buf[x2_addr(s, k2, k1lo3, cc)];
+            for (int b = 0; b < 8; ++b) v[s * 8 + b] = buf[x1_addr(k1lo + 8 * s, b, c)];
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            dft8<+1>(v + s * 8);
+#pragma unroll
+            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(s, k2, k1lo, c)] = v[s * 8 + k2];
+        }
+        wave_sync();
+        {
+            const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+            for (int s = 0; s < G; ++s)
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) v[s * 8 + cc] = buf[x2_addr(s, k2, k1lo3, cc)];
+        }
+        wave_sync();
+    } else {
+        // lane = 8 b1 + c1 = 8 k1lo + c, so both sides of exchange 1 index with (k1lo, c)
+        exchange<R1, true>(v, vbuf, [&](int k1) { return x1_addr(k1, k1lo, c); },
+                           [&](int r) { return x1_addr(k1lo + 8 * (r >> 3), r & 7, c); });
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            dft8<+1>(v + s * 8);
+#pragma unroll
+            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
+        }
+        const int k1lo3 = lane & 7, k2r = lane >> 3;
+        exchange<R1, true>(v, vbuf, [&](int r) { return x2_addr(r >> 3, r & 7, k1lo, c); },
+                           [&](int r) { return x2_addr(r >> 3, k2r, k1lo3, r & 7); });
     }
-    wave_sync();
 #pragma unroll
     for (int s = 0; s < G; ++s) dft8<+1>(v + s * 8);
 }
 
 // Inverse of fft_fwd without the 1/M scale and without the a-dependent untwist constants:
 // in: v in lane order; out: v[a] = M * z_{64a+lane} * (conj of the forward a-constant).
-template <int R1, typename T>
-__device__ __forceinline__ void fft_inv(C2<T>* v, const LaneTw<T, R1>& tw, C2<T>* buf, int lane) {
+template <int R1, typename T, bool HALF = false>
+__device__ __forceinline__ void fft_inv(C2<T>* v, const LaneTw<T, R1>& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
 #pragma unroll
     for (int s = 0; s < G; ++s) dft8<-1>(v + s * 8);
-    {
-        const int k1lo3 = lane & 7, k2 = lane >> 3;
+    const int c = lane & 7, k1lo = lane >> 3;
+    if constexpr (!HALF) {
+        C2<T>* buf = (C2<T>*)vbuf;
+        {
+            const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+            for (int s = 0; s < G; ++s)
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) buf[x2_addr(s, k2, k1lo3, cc)] = v[s * 8 + cc];
+        }
+        wave_sync();
 #pragma unroll
         for (int s = 0; s < G; ++s)
 #pragma unroll
-            for (int cc = 0; cc < 8; ++cc) buf[x2_addr(s, k2, k1lo3, cc)] = v[s * 8 + cc];
+            for (int k2 = 0; k2 < 8; ++k2) v[s * 8 + k2] = buf[x2_addr(s, k2, k1lo, c)];
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+#pragma unroll
+            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
+            dft8<-1>(v + s * 8);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo + 8 * s, b, c)] = v[s * 8 + b];
+        }
+        wave_sync();
+        {
+            const int b1 = lane >> 3, c1 = lane & 7;
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[x1_addr(k1, b1, c1)];
+        }
+        wave_sync();
+    } else {
+        const int k1lo3 = lane & 7, k2r = lane >> 3;
+        exchange<R1, true>(v, vbuf, [&](int r) { return x2_addr(r >> 3, k2r, k1lo3, r & 7); },
+                           [&](int r) { return x2_addr(r >> 3, r & 7, k1lo, c); });
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+#pragma unroll
+            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
+            dft8<-1>(v + s * 8);
+        }
+        exchange<R1, true>(v, vbuf, [&](int r) { return x1_addr(k1lo + 8 * (r >> 3), r & 7, c); },
+                           [&](int k1) { return x1_addr(k1, k1lo, c); });
     }
-    wave_sync();
-    const int c = lane & 7, k1lo = lane >> 3;
-#pragma unroll
-    for (int s = 0; s < G; ++s)
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) v[s * 8 + k2] = buf[x2_addr(s, k2, k1lo, c)];
-    wave_sync();
-#pragma unroll
-    for (int s = 0; s < G; ++s) {
-#pragma unroll
-        for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
-        dft8<-1>(v + s * 8);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo + 8 * s, b, c)] = v[s * 8 + b];
-    }
-    wave_sync();
-    {
-        const int b1 = lane >> 3, c1 = lane & 7;
-#pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[x1_addr(k1, b1, c1)];
-    }
-    wave_sync();
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
     dftR<R1, -1>(v);

@@ -392,6 +392,12 @@ extern "C" int tfhe_gpu_set_split_bits(int32_t bits) {
     return TFHE_OK;
 }
 
+extern "C" int tfhe_gpu_set_fp64_shared_rows(int32_t on) {
+    if (on != 0 && on != 1) return arg_fail("fp64 shared-rows flag must be 0 or 1");
+    set_boot64_shared(on != 0);
+    return TFHE_OK;
+}
+
 extern "C" int tfhe_gpu_set_default_precision(int32_t prec) {
     if (prec != TFHE_PREC_FP64 && prec != TFHE_PREC_SPLIT32) return arg_fail("unknown precision");
     ctx().default_prec = prec;

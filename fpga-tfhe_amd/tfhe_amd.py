@@ -82,6 +82,7 @@ def lib():
         "tfhe_gpu_sync": (ctypes.c_int, []),
         "tfhe_gpu_set_split_bits": (ctypes.c_int, [i32]),
         "tfhe_gpu_set_default_precision": (ctypes.c_int, [i32]),
+        "tfhe_gpu_set_fp64_shared_rows": (ctypes.c_int, [i32]),
         "tfhe_gpu_key_create": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32, P(_vp)]),
         "tfhe_gpu_key_upload": (ctypes.c_int, [_vp, i32, P(_vp)]),
         "tfhe_gpu_key_free": (ctypes.c_int, [_vp]),
@@ -208,6 +209,10 @@ def sync():
 
 def set_split_bits(b):
     _ck(lib().tfhe_gpu_set_split_bits(int(b)), "set_split_bits")
+
+
+def set_fp64_shared_rows(on):
+    _ck(lib().tfhe_gpu_set_fp64_shared_rows(1 if on else 0), "set_fp64_shared_rows")
 
 
 class DeviceKey:
