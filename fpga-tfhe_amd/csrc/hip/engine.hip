@@ -473,7 +473,7 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 
 template <int N, int L, int BG>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(TFHE_BOOT32_WAVES, TFHE_BOOT32_WAVES))) k_boot32(BootKArgs A) {
+__attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 ? TFHE_BOOT32_WAVES : 1))) k_boot32(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
     __shared__ C2<float> s_buf[xbuf_elems<M>()];

@@ -169,9 +169,10 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int X1_ROW = 72;
 __device__ __forceinline__ int x1_addr(int k1, int b, int c) { return k1 * X1_ROW + 8 * b + c; }
 // exchange 2: element (k1 = k1lo + 8 s, k2, c) -- written by lane c+8k1lo (register k2), read by
-// lane k1lo+8k2 (register c); the c ^ k1lo swizzle spreads the reads of one register over banks
+// lane k1lo+8k2 (register c).  The c ^ k1lo swizzle separates the k1lo of one read, and rows
+// padded to 72 separate its k2 (a 64-element stride would put every k2 on the same banks).
 __device__ __forceinline__ int x2_addr(int s, int k2, int k1lo, int c) {
-    return (s * 8 + k2) * 64 + 8 * k1lo + (c ^ k1lo);
+    return (s * 8 + k2) * X1_ROW + 8 * k1lo + (c ^ k1lo);
 }
 // LDS elements an exchange buffer needs for an M-point transform
 template <int M>

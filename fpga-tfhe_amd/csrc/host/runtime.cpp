@@ -56,7 +56,7 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::map<int, DevTwiddles> tw;
-    int split_bits = 24;
+    int split_bits = 0;          // 0: chosen per ring size (split_bits_for)
     int32_t default_prec = TFHE_PREC_FP64;
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -387,7 +387,7 @@ extern "C" int tfhe_gpu_sync(void) {
 }
 
 extern "C" int tfhe_gpu_set_split_bits(int32_t bits) {
-    if (bits < 8 || bits > 28) return arg_fail("split bits must be in [8, 28]");
+    if (bits != 0 && (bits < 8 || bits > 28)) return arg_fail("split bits must be 0 (default) or in [8, 28]");
     ctx().split_bits = bits;
     return TFHE_OK;
 }
@@ -403,6 +403,12 @@ extern "C" int tfhe_gpu_set_default_precision(int32_t prec) {
     ctx().default_prec = prec;
     return TFHE_OK;
 }
+
+// Split point of the FP32 key halves when none was set.  The hi product (digits x Khi summed over
+// (k+1) l N terms) must stay below ~2^21 for FP32 to round it to the exact integer: with
+// Khi < 2^(31-s), Bg/2 = 64 and N = 1024 that holds for s = 24; N = 2048 with Bg/2 = 128 grows the
+// sum by ~2^2, so its hi half is made 2 bits narrower.
+static int split_bits_for(const DevParams& p) { return p.N <= 1024 ? 24 : 26; }
 
 extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk, const int32_t* ksk,
                                    int32_t prec_mask, tfhe_gpu_key** out) {
@@ -422,7 +428,7 @@ extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk
     if ((rc = ensure_twiddles(p.N, &tw))) return rc;
     tfhe_gpu_key* k = new tfhe_gpu_key();
     k->p = p;
-    k->split_bits = c.split_bits;
+    k->split_bits = c.split_bits ? c.split_bits : split_bits_for(p);
     const int M = p.N / 2, kpl = 2 * p.l;
     const size_t npoly = (size_t)p.n * kpl * 2;
     const size_t bk_time_bytes = npoly * p.N * sizeof(int32_t);
