@@ -23,14 +23,36 @@ struct DevTwiddles {
     void* wa64 = nullptr;        // [R1]     C2<double>: w^{64a}
 };
 
+// One gate segment of a table-driven bootstrap launch (mode 3): jobs [s*seg_jobs, (s+1)*seg_jobs)
+// read x = (0, cst) + alpha*a[i] + beta*b[i] for i = job - s*seg_jobs (b may be null).
+struct SegDesc {
+    const int32_t* a;
+    const int32_t* b;
+    int32_t alpha, beta, cst, pad;
+};
+
+// One LWE linear job (no bootstrap): dst[i] = mul * src[i] + (0, add_b) over `count` samples, or
+// the trivial sample (0, add_b) when src is null.
+struct LinDesc {
+    const int32_t* src;
+    int32_t* dst;
+    int32_t mul, add_b;
+};
+
 // One launch of the fused bootstrap kernel.
 //   mode 0: job g reads x = (0, cst[s]) + alpha[s]*in_a[s][g'] + beta[s]*in_b[s][g'] from segment s
 //           (g' = g - seg_start[s]), modulus-switches it, builds the X^{-barb} test vector of mu,
 //           blind-rotates and writes the extracted sample u_out[g][0..N].
 //   mode 1: acc_io[g] (k+1 = 2 polys) is blind-rotated by bara_in[g][0..n) in place.
 //   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out optional.
+//   mode 3: as mode 0, with the gate inputs of job g taken from segs[g / seg_jobs].
+// form (FP64, N=1024): 0 = the process-wide choice, 1 = key-sharing workgroups, 2 = one bootstrap
+// per workgroup.
 struct BootLaunch {
     int mode = 0;
+    int form = 0;
+    const SegDesc* segs = nullptr;
+    int seg_jobs = 0;
     int count = 0;
     int nseg = 1;
     int seg_start[3] = {0, 0, 0};
@@ -62,15 +84,18 @@ bool params_supported(const DevParams& p);
 void set_boot64_shared(bool on);
 hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& L,
                             hipStream_t st);
-// KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.
+// KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.  Output g goes to out[g], or,
+// with out_tab, to sample g mod out_seg of the device array out_tab[g / out_seg].
 hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
-                            int32_t add_b, int count, hipStream_t st);
+                            int32_t add_b, int count, hipStream_t st, int32_t* const* out_tab = nullptr,
+                            int out_seg = 0);
 // time-domain bk [n][2l][2][N] (device) -> FP64 and/or split FP32 FFT-domain keys
 hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32_t* d_bk, void* bk64, void* bk32,
                           int split_bits, hipStream_t st);
 // KSK [kN][t][base][n+1] (device) -> [kN][t][base-1][NP]
 hipError_t launch_ksk_relayout(const DevParams& p, const int32_t* d_ksk_in, int32_t* d_ksk_out, int np,
                                hipStream_t st);
+hipError_t launch_lwe_linear(const LinDesc* d_descs, int ndesc, int count, int n, hipStream_t st);
 hipError_t launch_extract(int N, int32_t* u, const int32_t* acc, int count, hipStream_t st);
 hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* digits, const int32_t* acc,
                                    const int32_t* bara, int count, hipStream_t st);

@@ -51,19 +51,36 @@ template <int N>
 __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lane, int32_t* s_acc,
                                               int16_t* s_bara) {
     const int n = A.n;
-    if (A.L.mode == 0) {
+    if (A.L.mode == 0 || A.L.mode == 3) {
         constexpr int SH = 32 - (log2N<N>() + 1);
-        const int s = (A.L.nseg > 1 && g >= A.L.seg_start[1]) ? 1 : 0;
-        const int gl = g - A.L.seg_start[s];
-        const int32_t* xa = A.L.in_a[s] + (size_t)gl * (n + 1);
-        const int32_t* xb = A.L.in_b[s] ? A.L.in_b[s] + (size_t)gl * (n + 1) : nullptr;
-        const uint32_t al = (uint32_t)A.L.alpha[s], be = (uint32_t)A.L.beta[s];
+        const int32_t* xa;
+        const int32_t* xb;
+        uint32_t al, be;
+        int32_t cst;
+        if (A.L.mode == 0) {
+            const int s = (A.L.nseg > 1 && g >= A.L.seg_start[1]) ? 1 : 0;
+            const int gl = g - A.L.seg_start[s];
+            xa = A.L.in_a[s] + (size_t)gl * (n + 1);
+            xb = A.L.in_b[s] ? A.L.in_b[s] + (size_t)gl * (n + 1) : nullptr;
+            al = (uint32_t)A.L.alpha[s];
+            be = (uint32_t)A.L.beta[s];
+            cst = A.L.cst[s];
+        } else {
+            const int s = g / A.L.seg_jobs;
+            const int gl = g - s * A.L.seg_jobs;
+            const SegDesc D = A.L.segs[s];
+            xa = D.a + (size_t)gl * (n + 1);
+            xb = D.b ? D.b + (size_t)gl * (n + 1) : nullptr;
+            al = (uint32_t)D.alpha;
+            be = (uint32_t)D.beta;
+            cst = D.cst;
+        }
         for (int i = lane; i < n; i += 64) {
             uint32_t x = al * (uint32_t)xa[i];
             if (xb) x += be * (uint32_t)xb[i];
             s_bara[i] = (int16_t)((x + (1u << (SH - 1))) >> SH);
         }
-        uint32_t xbv = (uint32_t)A.L.cst[s] + al * (uint32_t)xa[n];
+        uint32_t xbv = (uint32_t)cst + al * (uint32_t)xa[n];
         if (xb) xbv += be * (uint32_t)xb[n];
         const int barb = (int)((xbv + (1u << (SH - 1))) >> SH);
         // b = X^{2N - barb} [mu, ..., mu] (mulByXaiOrCopy.cpp:310-326), a = 0 (tb :317)
@@ -84,7 +101,7 @@ __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lan
 // Sample extraction at index 0 (upstream tLweExtractLweSample, k = 1) or accumulator write-back.
 template <int N>
 __device__ __forceinline__ void boot_epilogue(const BootKArgs& A, int g, int lane, const int32_t* s_acc) {
-    if (A.L.mode == 0) {
+    if (A.L.mode == 0 || A.L.mode == 3) {
         int32_t* u = A.L.u_out + (size_t)g * (N + 1);
         for (int j = lane; j < N; j += 64) u[j] = (j == 0) ? s_acc[0] : (int32_t)(0u - (uint32_t)s_acc[N - j]);
         if (lane == 0) u[N] = s_acc[N];
@@ -521,7 +538,8 @@ constexpr int KS_IBLK = 16;
 template <int TKS, int KN, int NP, int C>
 __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
                                                         int32_t add_b, const int32_t* __restrict__ ksk,
-                                                        int32_t* __restrict__ out, int count, int n) {
+                                                        int32_t* __restrict__ out, int count, int n,
+                                                        int32_t* const* __restrict__ out_tab, int out_seg) {
     static_assert(KS_GATES % C == 0 && NP % C == 0 && KN % KS_IBLK == 0, "keyswitch tiling");
     // candidate rows, each padded by one uint4 so the 4 rows of a (j) start in distinct banks
     constexpr int KROW = C / 4 + 1;
@@ -617,7 +635,8 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
             for (int c = 0; c < C; ++c)
                 if (c0 + c == n) acc[c] += bterm;
         }
-        int32_t* o = out + (size_t)g * (n + 1);
+        // output g goes to out[g], or to sample g mod out_seg of the array out_tab[g / out_seg]
+        int32_t* o = out_tab ? out_tab[g / out_seg] + (size_t)(g % out_seg) * (n + 1) : out + (size_t)g * (n + 1);
 #pragma unroll
         for (int c = 0; c < C; ++c)
             if (c0 + c <= n) o[c0 + c] = (int32_t)acc[c];
@@ -685,6 +704,25 @@ __global__ void k_ksk_relayout(const int32_t* __restrict__ in, int32_t* __restri
         const size_t ij = r / 3;
         out[idx] = (c <= n) ? in[(ij * 4 + (hm1 + 1)) * (size_t)(n + 1) + c] : 0;
     }
+}
+
+// LWE linear jobs of a circuit level (NOT, COPY, constants): one workgroup per (desc, sample).
+__global__ void __launch_bounds__(64) k_lwe_linear(const LinDesc* __restrict__ descs, int count, int n) {
+    const int d = blockIdx.x / count, i = blockIdx.x - d * count;
+    const LinDesc D = descs[d];
+    int32_t* dst = D.dst + (size_t)i * (n + 1);
+    const int32_t* src = D.src ? D.src + (size_t)i * (n + 1) : nullptr;
+    for (int j = threadIdx.x; j <= n; j += 64) {
+        uint32_t v = src ? (uint32_t)D.mul * (uint32_t)src[j] : 0u;
+        if (j == n) v += (uint32_t)D.add_b;
+        dst[j] = (int32_t)v;
+    }
+}
+
+hipError_t launch_lwe_linear(const LinDesc* d_descs, int ndesc, int count, int n, hipStream_t st) {
+    if (ndesc <= 0 || count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lwe_linear, dim3(ndesc * count), dim3(64), 0, st, d_descs, count, n);
+    return hipGetLastError();
 }
 
 template <int N>
@@ -930,7 +968,8 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.t1 = tw.t1_fold64;
         A.t2 = tw.t2_64;
         if constexpr (N == 1024) {
-            if (Lc.mode != 2 && key.p.n <= B64W_MAXN && g_boot64_shared) {
+            const bool shared = Lc.form == 1 || (Lc.form == 0 && g_boot64_shared);
+            if (Lc.mode != 2 && key.p.n <= B64W_MAXN && shared) {
                 const int blocks = (Lc.count + B64W_GATES - 1) / B64W_GATES;
                 hipLaunchKernelGGL((k_boot64w<N, L, BG>), dim3(blocks), dim3(64 * B64W_GATES), 0, st, A);
                 return hipGetLastError();
@@ -957,23 +996,24 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
 
 template <int TKS, int KN, int NP>
 static hipError_t launch_ks_t(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
-                              int32_t add_b, int count, hipStream_t st) {
+                              int32_t add_b, int count, hipStream_t st, int32_t* const* out_tab, int out_seg) {
     constexpr int C = 32;
     if (key.ksk_np != NP || key.p.N != KN) return hipErrorInvalidValue;
     dim3 grid(NP / C, (count + KS_GATES - 1) / KS_GATES);
     hipLaunchKernelGGL((k_keyswitch<TKS, KN, NP, C>), grid, dim3(KS_GATES), 0, st, u1, u2, add_b, key.ksk, out, count,
-                       key.p.n);
+                       key.p.n, out_tab, out_seg);
     return hipGetLastError();
 }
 
 hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2, int32_t add_b,
-                            int count, hipStream_t st) {
+                            int count, hipStream_t st, int32_t* const* out_tab, int out_seg) {
     if (count <= 0) return hipSuccess;
+    if (out_tab && out_seg <= 0) return hipErrorInvalidValue;
     // the two supported parameter sets: (t, N, padded n+1) = (8, 1024, 640) and (10, 2048, 768)
     if (key.p.ks_t == 8 && key.p.N == 1024 && key.ksk_np == 640)
-        return launch_ks_t<8, 1024, 640>(key, out, u1, u2, add_b, count, st);
+        return launch_ks_t<8, 1024, 640>(key, out, u1, u2, add_b, count, st, out_tab, out_seg);
     if (key.p.ks_t == 10 && key.p.N == 2048 && key.ksk_np == 768)
-        return launch_ks_t<10, 2048, 768>(key, out, u1, u2, add_b, count, st);
+        return launch_ks_t<10, 2048, 768>(key, out, u1, u2, add_b, count, st, out_tab, out_seg);
     return hipErrorInvalidValue;
 }
 

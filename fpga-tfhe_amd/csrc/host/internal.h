@@ -2,7 +2,10 @@
 #pragma once
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
+
+#include <hip/hip_runtime.h>
 
 #include "tfhe_mi355x.h"
 
@@ -28,5 +31,25 @@ void register_keyset(const TFheGateBootstrappingCloudKeySet* bk);
 void forget_device_keys(const TFheGateBootstrappingCloudKeySet* bk);
 void key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 int32_t default_precision();
+
+// ---- runtime services shared with the circuit layer (all on the library's stream) ----
+struct DevKeyView;
+struct DevTwiddles;
+std::recursive_mutex& lib_mutex();
+int lib_ensure_init();                       // TFHE_OK or an error code (last error set)
+hipStream_t lib_stream();
+int lib_twiddles(int N, DevTwiddles** out);
+int lib_hip_fail(const char* what, hipError_t e);
+int lib_arg_fail(const char* what);
+int key_check_prec(tfhe_gpu_key* key, int32_t prec);
+DevKeyView key_view(const tfhe_gpu_key* key);
+// gate pre-combination x = (0, C) + alpha a + beta b of the two-input gates
+void gate_coefficients(int gate, int32_t* C, int32_t* alpha, int32_t* beta);
+// per-launch timing under tfhe_gpu_profile_enable (a no-op otherwise)
+struct LibProf {
+    explicit LibProf(const char* name);
+    ~LibProf();
+    void* impl;
+};
 
 }  // namespace tfhe

@@ -129,6 +129,9 @@ struct ProfScope {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ProfScope(Ctx& cc, const char* n) : c(cc), name(n) {
         if (!c.prof) return;
+        // launches being captured into a graph are not timed (their events would be graph nodes)
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(c.stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { e0 = e1 = nullptr; return; }
         (void)hipEventRecord(e0, c.stream);
     }
@@ -346,6 +349,19 @@ void forget_device_keys(const TFheGateBootstrappingCloudKeySet* bk) {
     c.by_cloud.erase(it);
     tfhe_gpu_key_free(k);
 }
+
+std::recursive_mutex& lib_mutex() { return ctx().mu; }
+int lib_ensure_init() { return ensure_init(); }
+hipStream_t lib_stream() { return ctx().stream; }
+int lib_twiddles(int N, DevTwiddles** out) { return ensure_twiddles(N, out); }
+int lib_hip_fail(const char* what, hipError_t e) { return hip_fail(what, e); }
+int lib_arg_fail(const char* what) { return arg_fail(what); }
+int key_check_prec(tfhe_gpu_key* key, int32_t prec) { return check_key_prec(key, prec); }
+DevKeyView key_view(const tfhe_gpu_key* key) { return key->view(); }
+void gate_coefficients(int gate, int32_t* C, int32_t* alpha, int32_t* beta) { gate_coeffs(gate, C, alpha, beta); }
+
+LibProf::LibProf(const char* name) : impl(new ProfScope(ctx(), name)) {}
+LibProf::~LibProf() { delete static_cast<ProfScope*>(impl); }
 
 }  // namespace tfhe
 

@@ -19,6 +19,8 @@ PREC_MASK_FP64, PREC_MASK_SPLIT32 = 1, 2
 GATES = {"NAND": 0, "AND": 1, "OR": 2, "XOR": 3, "XNOR": 4, "NOR": 5,
          "ANDNY": 6, "ANDYN": 7, "ORNY": 8, "ORYN": 9, "MUX": 10}
 MU = 1 << 29  # modSwitchToTorus32(1, 8)
+OP_NOT, OP_COPY = 32, 33
+CIRCUIT_GRAPH, CIRCUIT_AUTOTUNE = 1, 2
 
 
 class TfheError(RuntimeError):
@@ -110,6 +112,15 @@ def lib():
         "tfhe_gpu_profile_enable": (ctypes.c_int, [i32]),
         "tfhe_gpu_profile_reset": (ctypes.c_int, []),
         "tfhe_gpu_profile_read": (ctypes.c_int, [ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_int64)]),
+        "tfhe_circuit_create": (ctypes.c_int, [i32, P(_vp)]),
+        "tfhe_circuit_free": (ctypes.c_int, [_vp]),
+        "tfhe_circuit_add": (ctypes.c_int, [_vp, i32, i32, i32, i32, P(i32)]),
+        "tfhe_circuit_constant": (ctypes.c_int, [_vp, i32, P(i32)]),
+        "tfhe_circuit_output": (ctypes.c_int, [_vp, i32]),
+        "tfhe_circuit_info": (ctypes.c_int, [_vp, P(i32), P(i32), P(ctypes.c_int64), P(i32)]),
+        "tfhe_circuit_level_of": (ctypes.c_int, [_vp, i32, P(i32)]),
+        "tfhe_circuit_run": (ctypes.c_int, [_vp, _vp, i32, _vp, _vp, i32, i32]),
+        "tfhe_gpu_circuit_level": (ctypes.c_int, [_vp, i32, i32, _i32p, P(_vp), P(_vp), P(_vp), P(_vp), i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -377,3 +388,96 @@ def measure_valu_peak():
     a, b = ctypes.c_double(), ctypes.c_double()
     _ck(lib().tfhe_gpu_measure_valu_peak(ctypes.byref(a), ctypes.byref(b)), "measure_valu_peak")
     return a.value, b.value
+
+
+class Circuit:
+    """A netlist over wires (inputs 0..n_inputs-1, then one wire per added gate or constant),
+    evaluated level by level on the device over a batch of independent instances."""
+
+    def __init__(self, n_inputs):
+        self.ptr = ctypes.c_void_p()
+        _ck(lib().tfhe_circuit_create(int(n_inputs), ctypes.byref(self.ptr)), "circuit_create")
+        self.n_inputs = int(n_inputs)
+        self.n_outputs = 0
+
+    def add(self, op, a, b=-1, c=-1):
+        w = ctypes.c_int32()
+        code = {"NOT": OP_NOT, "COPY": OP_COPY}.get(op, None) if isinstance(op, str) else op
+        if code is None:
+            code = GATES[op]
+        _ck(lib().tfhe_circuit_add(self.ptr, code, int(a), int(b), int(c), ctypes.byref(w)), "circuit_add")
+        return w.value
+
+    def constant(self, value):
+        w = ctypes.c_int32()
+        _ck(lib().tfhe_circuit_constant(self.ptr, int(value), ctypes.byref(w)), "circuit_constant")
+        return w.value
+
+    def output(self, wire):
+        _ck(lib().tfhe_circuit_output(self.ptr, int(wire)), "circuit_output")
+        self.n_outputs += 1
+
+    def info(self):
+        lv, mx, no = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        nb = ctypes.c_int64()
+        _ck(lib().tfhe_circuit_info(self.ptr, ctypes.byref(lv), ctypes.byref(mx), ctypes.byref(nb),
+                                    ctypes.byref(no)), "circuit_info")
+        return {"levels": lv.value, "max_level_bootstraps": mx.value, "bootstraps": nb.value, "outputs": no.value}
+
+    def level_of(self, wire):
+        lv = ctypes.c_int32()
+        _ck(lib().tfhe_circuit_level_of(self.ptr, int(wire), ctypes.byref(lv)), "circuit_level_of")
+        return lv.value
+
+    def run_dev(self, key, d_in, d_out, batch, prec=PREC_FP64, graph=False, autotune=False):
+        flags = (CIRCUIT_GRAPH if graph else 0) | (CIRCUIT_AUTOTUNE if autotune else 0)
+        din = d_in.ptr if isinstance(d_in, DeviceBuffer) else d_in
+        dout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+        _ck(lib().tfhe_circuit_run(self.ptr, key.ptr, prec, din, dout, int(batch), flags), "circuit_run")
+
+    def run(self, key, inputs, prec=PREC_FP64, graph=False, autotune=False):
+        """inputs: int32 [n_inputs][batch][n+1] -> outputs [n_outputs][batch][n+1] (synchronous)."""
+        inputs = np.ascontiguousarray(inputs, np.int32)
+        batch = inputs.shape[1]
+        d_in = DeviceBuffer.from_array(inputs)
+        d_out = DeviceBuffer(max(4, self.n_outputs * batch * inputs.shape[2] * 4))
+        try:
+            self.run_dev(key, d_in, d_out, batch, prec, graph, autotune)
+            sync()
+            return d_out.to_array((self.n_outputs, batch, inputs.shape[2]))
+        finally:
+            d_in.free()
+            d_out.free()
+
+    def free(self):
+        if self.ptr:
+            lib().tfhe_circuit_free(self.ptr)
+            self.ptr = None
+
+
+def ripple_carry_adder(bits):
+    """Circuit of a `bits`-wide ripple-carry adder: inputs a_0..a_{bits-1}, b_0..b_{bits-1} (LSB
+    first); outputs s_0..s_{bits-1} and the carry out.  p_i = XOR(a_i, b_i), s_i = XOR(p_i, c_i),
+    c_{i+1} = MUX(p_i, c_i, a_i) with c_0 = 0."""
+    c = Circuit(2 * bits)
+    p = [c.add("XOR", i, bits + i) for i in range(bits)]
+    carry = c.constant(0)
+    sums = []
+    for i in range(bits):
+        sums.append(c.add("XOR", p[i], carry))
+        carry = c.add("MUX", p[i], carry, i)
+    for w in sums:
+        c.output(w)
+    c.output(carry)
+    return c
+
+
+def circuit_level(key, ops, a, b, c, out, batch, prec=PREC_FP64):
+    """One level of independent gates over device arrays (lists of DeviceBuffer or pointers)."""
+    n = len(ops)
+    ptrs = lambda xs: (_vp * n)(*[(x.ptr if isinstance(x, DeviceBuffer) else x) if x is not None else None
+                                  for x in xs])
+    codes = np.array([GATES[o] if isinstance(o, str) else o for o in ops], np.int32)
+    cc = ptrs(c) if c is not None else None
+    _ck(lib().tfhe_gpu_circuit_level(key.ptr, prec, n, i32p(codes), ptrs(a), ptrs(b), cc, ptrs(out), int(batch)),
+        "circuit_level")

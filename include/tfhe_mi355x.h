@@ -376,6 +376,48 @@ int tfhe_gpu_profile_enable(int32_t on);
 int tfhe_gpu_profile_reset(void);
 int tfhe_gpu_profile_read(const char* kernel, double* total_ms, int64_t* launches);
 
+/* ---------------------------------------------------------------------------------------------
+ * Circuits.  The reference evaluates a circuit one upstream gate call at a time
+ * (test_bench/server/cloud.c:8-40 calls bootsXNOR, bootsMUX and bootsOR in host loops); these
+ * entry points replace such loops.  A netlist is levelised once, and every level becomes one
+ * bootstrap launch over all of its gates and all instances of the batch, plus at most two
+ * keyswitch launches.  NOT and COPY (upstream bootsNOT / bootsCOPY, no bootstrap) are folded into
+ * the coefficients of the gates that read them.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct tfhe_circuit tfhe_circuit;
+enum { TFHE_OP_NOT = 32, TFHE_OP_COPY = 33 };
+enum { TFHE_CIRCUIT_GRAPH = 1, TFHE_CIRCUIT_AUTOTUNE = 2 };
+
+/* wires 0 .. n_inputs-1 are the inputs; each added gate or constant creates the next wire */
+int tfhe_circuit_create(int32_t n_inputs, tfhe_circuit** out);
+int tfhe_circuit_free(tfhe_circuit* c);
+/* op: TFHE_GATE_* (inputs a, b; MUX(a, b, c) = a ? b : c as upstream bootsMUX), TFHE_OP_NOT or
+ * TFHE_OP_COPY (input a) */
+int tfhe_circuit_add(tfhe_circuit* c, int32_t op, int32_t a, int32_t b, int32_t c_in, int32_t* out_wire);
+/* a trivial ciphertext of value 0 or 1 (upstream bootsCONSTANT) */
+int tfhe_circuit_constant(tfhe_circuit* c, int32_t value, int32_t* out_wire);
+/* appends a circuit output (outputs are numbered in call order) */
+int tfhe_circuit_output(tfhe_circuit* c, int32_t wire);
+/* bootstrapped depth, the largest number of bootstraps in one level and in total, per instance */
+int tfhe_circuit_info(tfhe_circuit* c, int32_t* n_levels, int32_t* max_level_bootstraps, int64_t* n_bootstraps,
+                      int32_t* n_outputs);
+/* bootstrapped level at which a wire is available (0 for inputs and constants) */
+int tfhe_circuit_level_of(tfhe_circuit* c, int32_t wire, int32_t* level);
+/* Evaluates `batch` independent instances.  d_in: device [n_inputs][batch][n+1], d_out: device
+ * [n_outputs][batch][n+1].  Asynchronous on the library stream (tfhe_gpu_sync waits).  flags:
+ * TFHE_CIRCUIT_GRAPH captures the launch sequence into a HIP graph on the first run of a binding
+ * (key, precision, batch, d_in, d_out) and replays it afterwards; TFHE_CIRCUIT_AUTOTUNE times both
+ * FP64 launch forms for every level width not seen before and keeps the faster.  Runs of one
+ * circuit must not overlap (its wire storage is shared). */
+int tfhe_circuit_run(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec, const int32_t* d_in, int32_t* d_out,
+                     int32_t batch, int32_t flags);
+/* One level of independent gates, synchronous: gate i (TFHE_GATE_*) reads the device arrays a[i],
+ * b[i] (and c[i] for MUX), each [batch][n+1], and writes out[i].  The pointer arrays are host
+ * arrays of device pointers. */
+int tfhe_gpu_circuit_level(tfhe_gpu_key* key, int32_t prec, int32_t n_gates, const int32_t* ops,
+                           const int32_t* const* a, const int32_t* const* b, const int32_t* const* c,
+                           int32_t* const* out, int32_t batch);
+
 #ifdef __cplusplus
 }
 #endif
