@@ -7,7 +7,9 @@ seed 0; uniformly random input bits (numpy PCG64, seed 1).
 
 Prints one JSON line.  `value` is the split-FP32 rate; the FP64 rate, their ratio, the chained
 10^6-gate decryption audit, the roofline of the bootstrap kernel (timed with HIP events on the
-library stream) and the CPU baseline (the oracle's FP64 FFT path on the host cores) ride along.
+library stream) and the CPU baseline (the oracle's FP64 FFT path on the host cores) ride along,
+with two secondary workloads: 4096 independent 32-bit ripple-carry adders evaluated as one
+levelised circuit (graph-replayed), and 8192 MUX + 8192 XOR gates at the N=2048 parameter set.
 """
 import argparse
 import json
@@ -38,6 +40,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prec", choices=["both", "split32", "fp64"], default="both")
+    ap.add_argument("--adders", type=int, default=4096, help="32-bit adders per circuit run (0: skip)")
+    ap.add_argument("--n2048-gates", type=int, default=8192,
+                    help="MUX and XOR gates each at the N=2048 set (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -128,6 +133,101 @@ def precision_report(T, dk, rng, count=64):
             "fp64_max_dist_to_integer": dist}
 
 
+def adders_report(T, dk, ks, count, reps=2):
+    """`count` independent 32-bit ripple-carry adders as one circuit over a batch of `count`
+    instances: 33 bootstrapped levels (one of 32 XOR, then MUX + XOR along the carry chain), each
+    one bootstrap launch and at most two keyswitch launches, replayed from a HIP graph with the FP64
+    launch form autotuned per level width.  Operands from numpy PCG64 seed 3; sums checked against
+    plain addition after GPU decryption."""
+    rng = np.random.default_rng(3)
+    x = rng.integers(0, 2**32, count, dtype=np.uint64)
+    y = rng.integers(0, 2**32, count, dtype=np.uint64)
+    bits = np.array([[(int(v) >> i) & 1 for v in vals] for vals in (x, y) for i in range(32)], np.int32)
+    t = time.perf_counter()
+    ct = np.stack([ks.encrypt(bits[i]) for i in range(64)])
+    log(f"[bench] adders: encrypted {bits.size} input bits in {time.perf_counter() - t:.1f} s")
+    c = T.ripple_carry_adder(32)
+    info = c.info()
+    d_in = T.DeviceBuffer.from_array(ct)
+    del ct
+    d_out = T.DeviceBuffer(33 * count * (n + 1) * 4)
+    res = {"workload": f"{count} independent 32-bit ripple-carry adders (p=XOR(a,b), s=XOR(p,c), "
+                       f"c'=MUX(p,c,a)), one circuit run over a batch of {count}",
+           "levels": info["levels"], "bootstraps_per_adder": info["bootstraps"]}
+    try:
+        for name, prec in (("split32", T.PREC_SPLIT32), ("fp64", T.PREC_FP64)):
+            c.run_dev(dk, d_in, d_out, count, prec=prec, graph=True, autotune=True)   # tune + capture
+            T.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                c.run_dev(dk, d_in, d_out, count, prec=prec, graph=True, autotune=True)
+            T.sync()
+            sec = (time.perf_counter() - t0) / reps
+            dec = np.stack([T.decrypt_dev(d_out.ptr.value + i * count * (n + 1) * 4, ks.lwe_key, count)
+                            for i in range(33)])
+            s_ = [int(sum(int(dec[i][j]) << i for i in range(32))) for j in range(count)]
+            ok = s_ == [int((a + b) % 2**32) for a, b in zip(x, y)] and \
+                list(dec[32]) == [int((a + b) >> 32) for a, b in zip(x, y)]
+            res[name] = {"ms_per_run": round(sec * 1e3, 2), "adders_per_s": round(count / sec, 1),
+                         "bootstraps_per_s": round(count * info["bootstraps"] / sec, 1), "correct": bool(ok)}
+            log(f"[bench] adders {name}: {res[name]}")
+    finally:
+        d_in.free()
+        d_out.free()
+        c.free()
+    return res
+
+
+def n2048_report(T, gates, steps=2):
+    """N=2048 set (n=750, l=3, Bgbit=8, ks t=10): `gates` MUX and `gates` XOR on random bits per
+    step (one launch each), both paths; decryption failures counted over every step, and the split
+    path's external-product error against the FP64 path."""
+    t = time.perf_counter()
+    ks = T.KeySet(2048, seed=(0,))
+    dk = T.DeviceKey.from_keyset(ks)
+    log(f"[bench] N=2048 keys ready in {time.perf_counter() - t:.1f} s")
+    rng = np.random.default_rng(5)
+    nn = ks.fp.n
+    bits = rng.integers(0, 2, (3, gates)).astype(np.int32)
+    bufs = [T.DeviceBuffer.from_array(ks.encrypt(bits[i])) for i in range(3)]
+    outs = [T.DeviceBuffer(gates * (nn + 1) * 4) for _ in range(2)]
+    want_mux = np.where(bits[0] == 1, bits[1], bits[2])
+    want_xor = bits[0] ^ bits[1]
+    res = {"workload": f"{gates} MUX + {gates} XOR per step, N=2048 n=750 l=3 Bgbit=8 ks t=10",
+           "bootstraps_per_step": 3 * gates}
+    try:
+        for name, prec in (("split32", T.PREC_SPLIT32), ("fp64", T.PREC_FP64)):
+            fails = 0
+            dk.gate_dev("MUX", outs[0].ptr, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, gates, prec=prec)
+            T.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                dk.gate_dev("MUX", outs[0].ptr, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, gates, prec=prec)
+                dk.gate_dev("XOR", outs[1].ptr, bufs[0].ptr, bufs[1].ptr, None, gates, prec=prec)
+            T.sync()
+            sec = (time.perf_counter() - t0) / steps
+            fails += int((T.decrypt_dev(outs[0], ks.lwe_key, gates) != want_mux).sum())
+            fails += int((T.decrypt_dev(outs[1], ks.lwe_key, gates) != want_xor).sum())
+            res[name] = {"ms_per_step": round(sec * 1e3, 2), "gates_per_s": round(2 * gates / sec, 1),
+                         "bootstraps_per_s": round(3 * gates / sec, 1), "failures": fails,
+                         "gates_checked": 2 * gates}
+            log(f"[bench] N=2048 {name}: {res[name]}")
+        acc = rng.integers(-2**31, 2**31, (32, 2, 2048), dtype=np.int64).astype(np.int32)
+        rows = rng.integers(0, nn, 32).astype(np.int32)
+        r64 = dk.extern_product(acc, rows, prec=T.PREC_FP64)
+        r32 = dk.extern_product(acc, rows, prec=T.PREC_SPLIT32)
+        err = np.abs(((r32.astype(np.int64) - r64.astype(np.int64) + 2**31) % 2**32) - 2**31)
+        res["split32_extern_product_error"] = {"samples": int(err.size), "max": int(err.max()),
+                                               "rms": round(float(np.sqrt((err.astype(float) ** 2).mean())), 1),
+                                               "unit": "2^-32 torus"}
+    finally:
+        for b in bufs + outs:
+            b.free()
+        dk.free()
+        ks.free()
+    return res
+
+
 def cpu_baseline(ks, batch_bits, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -199,6 +299,9 @@ def main():
     prec_rep = precision_report(T, dk, np.random.default_rng(3)) if len(precs) == 2 else None
     log(f"[bench] split32 external-product error {prec_rep}")
 
+    adders = adders_report(T, dk, ks, args.adders) if args.adders > 0 and args.prec == "both" else None
+    n2048 = n2048_report(T, args.n2048_gates) if args.n2048_gates > 0 and args.prec == "both" else None
+
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -242,6 +345,8 @@ def main():
                      "traffic": traffic, "kernel": f"bootstrap_{head}", "flop_per_gate": FLOP_PER_GATE,
                      "measured_valu_peak": {"fp64": round(vp64, 1), "fp32_unpacked": round(vp32, 1)}},
         "cpu_baseline": cpu,
+        "circuits_32bit_adders": adders,
+        "n2048": n2048,
         "device": dev,
     }
     print(json.dumps(line), flush=True)
