@@ -32,6 +32,19 @@ void forget_device_keys(const TFheGateBootstrappingCloudKeySet* bk);
 void key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 int32_t default_precision();
 
+// ---- key-set containers shared by keygen and file import (tfhe_api.cpp, tfhe_io.cpp) ----
+TFheGateBootstrappingCloudKeySet alloc_cloud_keyset(const TFheGateBootstrappingParameterSet* params);
+LweKey* alloc_lwe_key(const LweParams* params);
+TGswKey* alloc_tgsw_key(const TGswParams* params);
+TFheGateBootstrappingParameterSet* alloc_params(const tfhe_flat_params& f, double lwe_alpha_min, double lwe_alpha_max,
+                                                double tlwe_alpha_min, double tlwe_alpha_max);
+// ownership records of objects read from files: a standalone cloud key set owns its parameter
+// set; a secret key set read from a file owns its parameter set.  take_* removes the record.
+void note_standalone_cloud(const TFheGateBootstrappingCloudKeySet* c, const TFheGateBootstrappingParameterSet* p);
+bool take_standalone_cloud(const TFheGateBootstrappingCloudKeySet* c, const TFheGateBootstrappingParameterSet** p);
+void note_owned_params(const TFheGateBootstrappingSecretKeySet* k, const TFheGateBootstrappingParameterSet* p);
+const TFheGateBootstrappingParameterSet* take_owned_params(const TFheGateBootstrappingSecretKeySet* k);
+
 // ---- runtime services shared with the circuit layer (all on the library's stream) ----
 struct DevKeyView;
 struct DevTwiddles;

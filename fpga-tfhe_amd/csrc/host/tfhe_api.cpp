@@ -463,6 +463,27 @@ extern "C" TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secr
     for (int q = 0; q < k; ++q)
         for (int j = 0; j < N; ++j) tgsw_key->tlwe_key.key[q].coefs[j] = uniform_bit();
 
+    TFheGateBootstrappingCloudKeySet cloud = tfhe::alloc_cloud_keyset(params);
+    LweBootstrappingKey* bk = const_cast<LweBootstrappingKey*>(cloud.bk);
+    std::vector<int32_t> extracted((size_t)k * N);
+    for (int q = 0; q < k; ++q)
+        for (int j = 0; j < N; ++j) extracted[(size_t)q * N + j] = tgsw_key->tlwe_key.key[q].coefs[j];
+    create_ksk(bk->ks, extracted.data(), lwe_key);
+    for (int i = 0; i < n; ++i) tgsw_encrypt_int(bk->bk + i, lwe_key->key[i], accum->alpha_min, tgsw_key);
+
+    TFheGateBootstrappingSecretKeySet* ks = new TFheGateBootstrappingSecretKeySet{params, lwe_key, tgsw_key, cloud};
+    register_keyset(&ks->cloud);
+    return ks;
+}
+
+namespace tfhe {
+
+// The containers of a cloud key set (time-domain bk, keyswitch key, FFT-domain stubs), unfilled.
+TFheGateBootstrappingCloudKeySet alloc_cloud_keyset(const TFheGateBootstrappingParameterSet* params) {
+    const LweParams* in_out = params->in_out_params;
+    const TGswParams* bkp = params->tgsw_params;
+    const TLweParams* accum = bkp->tlwe_params;
+    const int n = in_out->n, N = accum->N, k = accum->k;
     LweBootstrappingKey* bk = new LweBootstrappingKey();
     bk->in_out_params = in_out;
     bk->bk_params = bkp;
@@ -471,13 +492,6 @@ extern "C" TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secr
     bk->ks = new_LweKeySwitchKey(k * N, params->ks_t, params->ks_basebit, in_out);
     bk->bk = new TGswSample[n];
     for (int i = 0; i < n; ++i) init_tgsw(bk->bk + i, bkp);
-
-    std::vector<int32_t> extracted((size_t)k * N);
-    for (int q = 0; q < k; ++q)
-        for (int j = 0; j < N; ++j) extracted[(size_t)q * N + j] = tgsw_key->tlwe_key.key[q].coefs[j];
-    create_ksk(bk->ks, extracted.data(), lwe_key);
-    for (int i = 0; i < n; ++i) tgsw_encrypt_int(bk->bk + i, lwe_key->key[i], accum->alpha_min, tgsw_key);
-
     LweBootstrappingKeyFFT* bkfft = new LweBootstrappingKeyFFT();
     bkfft->in_out_params = in_out;
     bkfft->bk_params = bkp;
@@ -487,12 +501,20 @@ extern "C" TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secr
     TGswSampleFFT* samples = new TGswSampleFFT[n];
     for (int i = 0; i < n; ++i) init_tgsw_fft(samples + i, bkp, (void*)bkfft);
     bkfft->bkFFT = samples;
-
-    TFheGateBootstrappingSecretKeySet* ks = new TFheGateBootstrappingSecretKeySet{
-        params, lwe_key, tgsw_key, TFheGateBootstrappingCloudKeySet{params, bk, bkfft}};
-    register_keyset(&ks->cloud);
-    return ks;
+    return TFheGateBootstrappingCloudKeySet{params, bk, bkfft};
 }
+
+LweKey* alloc_lwe_key(const LweParams* params) { return new_LweKey(params); }
+TGswKey* alloc_tgsw_key(const TGswParams* params) { return new_TGswKey(params); }
+TFheGateBootstrappingParameterSet* alloc_params(const tfhe_flat_params& f, double lwe_alpha_min, double lwe_alpha_max,
+                                                double tlwe_alpha_min, double tlwe_alpha_max) {
+    LweParams* in_out = new_LweParams(f.n, lwe_alpha_min, lwe_alpha_max);
+    TLweParams* accum = new_TLweParams(f.N, f.k, tlwe_alpha_min, tlwe_alpha_max);
+    TGswParams* bkp = new_TGswParams(f.l, f.Bgbit, accum);
+    return new_TFheGateBootstrappingParameterSet(f.ks_t, f.ks_basebit, in_out, bkp);
+}
+
+}  // namespace tfhe
 
 static void destroy_cloud(const TFheGateBootstrappingCloudKeySet* c) {
     forget_device_keys(c);
@@ -519,13 +541,23 @@ extern "C" void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSec
     destroy_cloud(&keyset->cloud);
     delete_LweKey(const_cast<LweKey*>(keyset->lwe_key));
     delete_TGswKey(const_cast<TGswKey*>(keyset->tgsw_key));
+    const TFheGateBootstrappingParameterSet* owned = tfhe::take_owned_params(keyset);
     delete keyset;
+    if (owned) delete_gate_bootstrapping_parameters(const_cast<TFheGateBootstrappingParameterSet*>(owned));
 }
 
-// A cloud key set is embedded in its secret key set by this library (as upstream does), so
-// deleting only the cloud part releases its device keys; the rest goes with the secret key set.
+// A cloud key set made by keygen is embedded in its secret key set (as upstream does), so deleting
+// only the cloud part releases its device keys and the rest goes with the secret key set.  A cloud
+// key set read from a file stands alone and owns everything, its parameter set included.
 extern "C" void delete_gate_bootstrapping_cloud_keyset(TFheGateBootstrappingCloudKeySet* keyset) {
-    if (keyset) forget_device_keys(keyset);
+    if (!keyset) return;
+    forget_device_keys(keyset);
+    const TFheGateBootstrappingParameterSet* owned = nullptr;
+    if (tfhe::take_standalone_cloud(keyset, &owned)) {
+        destroy_cloud(keyset);
+        delete keyset;
+        delete_gate_bootstrapping_parameters(const_cast<TFheGateBootstrappingParameterSet*>(owned));
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -579,6 +611,13 @@ extern "C" int tfhe_keyset_export_flat(const TFheGateBootstrappingSecretKeySet* 
                                                sizeof(int32_t) * tp->N);
     if (bk) flatten_bk(key->cloud.bk, bk);
     if (ksk) flatten_ksk(key->cloud.bk->ks, ksk);
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cloud_export_flat(const TFheGateBootstrappingCloudKeySet* cloud, int32_t* bk, int32_t* ksk) {
+    if (!cloud || !cloud->bk) return TFHE_E_ARG;
+    if (bk) flatten_bk(cloud->bk, bk);
+    if (ksk) flatten_ksk(cloud->bk->ks, ksk);
     return TFHE_OK;
 }
 

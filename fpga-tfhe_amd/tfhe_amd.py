@@ -75,6 +75,16 @@ def lib():
         "dtot32": (i32, [ctypes.c_double]),
         "tfhe_keyset_export_flat": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
         "tfhe_flat_params_of": (ctypes.c_int, [_vp, P(FlatParams)]),
+        "tfhe_cloud_export_flat": (ctypes.c_int, [_vp, _i32p, _i32p]),
+        "export_tfheGateBootstrappingParameterSet_toFile": (None, [_vp, _vp]),
+        "new_tfheGateBootstrappingParameterSet_fromFile": (_vp, [_vp]),
+        "export_tfheGateBootstrappingSecretKeySet_toFile": (None, [_vp, _vp]),
+        "new_tfheGateBootstrappingSecretKeySet_fromFile": (_vp, [_vp]),
+        "export_tfheGateBootstrappingCloudKeySet_toFile": (None, [_vp, _vp]),
+        "new_tfheGateBootstrappingCloudKeySet_fromFile": (_vp, [_vp]),
+        "export_gate_bootstrapping_ciphertext_toFile": (None, [_vp, P(LweSample), _vp]),
+        "import_gate_bootstrapping_ciphertext_fromFile": (None, [_vp, P(LweSample), _vp]),
+        "delete_gate_bootstrapping_cloud_keyset": (None, [_vp]),
         "tfhe_encrypt_bits_flat": (ctypes.c_int, [_vp, _i32p, i32, _i32p]),
         "tfhe_decrypt_bits_flat": (ctypes.c_int, [_vp, _i32p, i32, _i32p]),
         # flat ABI

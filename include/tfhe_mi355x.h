@@ -26,6 +26,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -257,6 +258,25 @@ void lweKeySwitch(LweSample* result, const LweKeySwitchKey* ks, const LweSample*
 void tLweExtractLweSample(LweSample* result, const TLweSample* x, const LweParams* params,
                           const TLweParams* rparams);
 
+/* key-set, parameter-set and ciphertext files: the upstream tfhe_io.h names as called at
+ * test_bench/client/alice.c:19-27,49-54, test_bench/server/cloud.c:42-44,54-57,65-67 and
+ * test_bench/client/verify.c:8-10,19-22.  The byte format is this library's own (documented in
+ * fpga-tfhe_amd/csrc/host/tfhe_io.cpp): upstream's is not available to match, so a file written by
+ * upstream tfhe is not readable here, nor the reverse.  The new_* readers return NULL on a
+ * malformed file (tfhe_gpu_last_error says why); the other calls abort, as upstream does.  A cloud
+ * key set read from a file owns its parameter set; delete_gate_bootstrapping_cloud_keyset frees
+ * it. */
+void export_tfheGateBootstrappingParameterSet_toFile(FILE* F, const TFheGateBootstrappingParameterSet* params);
+TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile(FILE* F);
+void export_tfheGateBootstrappingSecretKeySet_toFile(FILE* F, const TFheGateBootstrappingSecretKeySet* key);
+TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromFile(FILE* F);
+void export_tfheGateBootstrappingCloudKeySet_toFile(FILE* F, const TFheGateBootstrappingCloudKeySet* bk);
+TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromFile(FILE* F);
+void export_gate_bootstrapping_ciphertext_toFile(FILE* F, const LweSample* sample,
+                                                 const TFheGateBootstrappingParameterSet* params);
+void import_gate_bootstrapping_ciphertext_fromFile(FILE* F, LweSample* sample,
+                                                   const TFheGateBootstrappingParameterSet* params);
+
 /* ======================================================================================= */
 /* 2. flat batched C ABI                                                                   */
 /* ======================================================================================= */
@@ -315,6 +335,8 @@ int tfhe_gpu_set_fp64_shared_rows(int32_t on);
  * be NULL): LWE key [n], TLWE key [k][N], time-domain bk, keyswitch key (layouts above) */
 int tfhe_keyset_export_flat(const TFheGateBootstrappingSecretKeySet* key, int32_t* lwe_key, int32_t* tlwe_key,
                             int32_t* bk, int32_t* ksk);
+/* flat views of a cloud key set (made by keygen or read from a file): time-domain bk, KSK */
+int tfhe_cloud_export_flat(const TFheGateBootstrappingCloudKeySet* cloud, int32_t* bk, int32_t* ksk);
 int tfhe_flat_params_of(const TFheGateBootstrappingParameterSet* params, tfhe_flat_params* out);
 /* bootsSymEncrypt / bootsSymDecrypt over flat arrays [count][n+1] (same draw order as a loop of
  * bootsSymEncrypt calls) */
