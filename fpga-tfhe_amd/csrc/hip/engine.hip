@@ -33,7 +33,8 @@ struct BootKArgs {
     const void* bk;
     const void* t1;
     const void* t2;
-    const void* twist64;
+    const void* twist32;
+    const void* untw64;
     const void* wa64;
     int n;
     int split_bits;
@@ -58,13 +59,23 @@ __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lan
         uint32_t al, be;
         int32_t cst;
         if (A.L.mode == 0) {
-            const int s = (A.L.nseg > 1 && g >= A.L.seg_start[1]) ? 1 : 0;
-            const int gl = g - A.L.seg_start[s];
-            xa = A.L.in_a[s] + (size_t)gl * (n + 1);
-            xb = A.L.in_b[s] ? A.L.in_b[s] + (size_t)gl * (n + 1) : nullptr;
-            al = (uint32_t)A.L.alpha[s];
-            be = (uint32_t)A.L.beta[s];
-            cst = A.L.cst[s];
+            // both entries are read and one is picked with a mask: a select of the two would be
+            // folded into an indexed load, which copies the argument arrays to scratch
+            const bool s1 = A.L.nseg > 1 && g >= A.L.seg_start[1];
+            const uint32_t m = 0u - (uint32_t)s1;
+            auto pick = [m](int32_t v0, int32_t v1) { return (int32_t)(((uint32_t)v0 & ~m) | ((uint32_t)v1 & m)); };
+            auto pickp = [m](const int32_t* p0, const int32_t* p1) {
+                const uint64_t mm = (uint64_t)(int64_t)(int32_t)m;
+                return (const int32_t*)(((uint64_t)p0 & ~mm) | ((uint64_t)p1 & mm));
+            };
+            const int gl = g - pick(A.L.seg_start[0], A.L.seg_start[1]);
+            const int32_t* ia = pickp(A.L.in_a[0], A.L.in_a[1]);
+            const int32_t* ib = pickp(A.L.in_b[0], A.L.in_b[1]);
+            xa = ia + (size_t)gl * (n + 1);
+            xb = ib ? ib + (size_t)gl * (n + 1) : nullptr;
+            al = (uint32_t)pick(A.L.alpha[0], A.L.alpha[1]);
+            be = (uint32_t)pick(A.L.beta[0], A.L.beta[1]);
+            cst = pick(A.L.cst[0], A.L.cst[1]);
         } else {
             const int s = g / A.L.seg_jobs;
             const int gl = g - s * A.L.seg_jobs;
@@ -402,15 +413,16 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
 
 // ---------------------------------------------------------------------------------------------
 // Split-FP32 path.  Each key coefficient K = Khi * 2^s + Klo with Klo in [-2^(s-1), 2^(s-1));
-// both halves are stored as FP32 transforms.  Digits are twisted in FP64 and rounded to FP32,
-// transformed and multiplied in FP32, and the hi and lo products are inverse-transformed
-// separately.  Untwist and rounding run in FP64; the hi product is small enough to round to the
-// exact integer, so the 2^s scaling does not amplify FP32 error.
+// both halves are stored as FP32 transforms.  Digits (small integers, exact in FP32) are twisted
+// in FP32, transformed and multiplied in FP32, and the hi and lo products are inverse-transformed
+// separately.  Untwist and rounding run in FP64 with the 1/M scale folded into the untwist
+// factors, which are re-read each step instead of held in registers; the hi product is small
+// enough to round to the exact integer, so the 2^s scaling does not amplify FP32 error.
 // ---------------------------------------------------------------------------------------------
 template <int N, int L, int BG, bool ROT>
 __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, const float4* __restrict__ kr,
-                                       const LaneTw<float, N / 128>& tw, const C2<double>* tws, int lane,
-                                       int split_bits, double* raw) {
+                                       const LaneTw<float, N / 128>& tw, const C2<float>* tws,
+                                       const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
@@ -431,12 +443,9 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
         for (int lvl = 0; lvl < L; ++lvl) {
             C2<float> v[R1];
 #pragma unroll
-            for (int aa = 0; aa < R1; ++aa) {
-                C2<double> z{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
-                z = cmul(z, tws[aa]);
-                v[aa] = C2<float>{(float)z.x, (float)z.y};
-            }
-            fft_fwd<R1>(v, tw, s_buf, lane);
+            for (int aa = 0; aa < R1; ++aa)
+                v[aa] = cmul(C2<float>{(float)digit<BG>(u[aa], lvl), (float)digit<BG>(u[aa + R1], lvl)}, tws[aa]);
+            fft_fwd<R1, float, false, true>(v, tw, s_buf, lane);
             const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
@@ -452,21 +461,23 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             }
         }
     }
-    fft_inv<R1>(h0, tw, s_buf, lane);
-    fft_inv<R1>(l0, tw, s_buf, lane);
-    fft_inv<R1>(h1, tw, s_buf, lane);
-    fft_inv<R1>(l1, tw, s_buf, lane);
-    const double sc = 1.0 / (double)M;
+    fft_inv<R1, float, false, true>(h0, tw, s_buf, lane);
+    fft_inv<R1, float, false, true>(l0, tw, s_buf, lane);
+    fft_inv<R1, float, false, true>(h1, tw, s_buf, lane);
+    fft_inv<R1, float, false, true>(l1, tw, s_buf, lane);
     const double hs = ldexp(1.0, split_bits);
+    // the untwist factors are re-read every step (an opaque offset keeps the compiler from
+    // hoisting them out of the blind-rotation loop into 32 long-lived registers)
+    int uoff = lane * R1;
+    asm volatile("" : "+v"(uoff));
 #pragma unroll
     for (int aa = 0; aa < R1; ++aa) {
         const int j = lane + 64 * aa;
-        C2<double> zh0 = cmulc(C2<double>{(double)h0[aa].x, (double)h0[aa].y}, tws[aa]);
-        C2<double> zl0 = cmulc(C2<double>{(double)l0[aa].x, (double)l0[aa].y}, tws[aa]);
-        C2<double> zh1 = cmulc(C2<double>{(double)h1[aa].x, (double)h1[aa].y}, tws[aa]);
-        C2<double> zl1 = cmulc(C2<double>{(double)l1[aa].x, (double)l1[aa].y}, tws[aa]);
-        zh0.x *= sc; zh0.y *= sc; zl0.x *= sc; zl0.y *= sc;
-        zh1.x *= sc; zh1.y *= sc; zl1.x *= sc; zl1.y *= sc;
+        const C2<double> w = untw[uoff + aa];
+        C2<double> zh0 = cmulc(C2<double>{(double)h0[aa].x, (double)h0[aa].y}, w);
+        C2<double> zl0 = cmulc(C2<double>{(double)l0[aa].x, (double)l0[aa].y}, w);
+        C2<double> zh1 = cmulc(C2<double>{(double)h1[aa].x, (double)h1[aa].y}, w);
+        C2<double> zl1 = cmulc(C2<double>{(double)l1[aa].x, (double)l1[aa].y}, w);
         const uint32_t r00 = (round_mod32(zh0.x) << split_bits) + round_mod32(zl0.x);
         const uint32_t r01 = (round_mod32(zh0.y) << split_bits) + round_mod32(zl0.y);
         const uint32_t r10 = (round_mod32(zh1.x) << split_bits) + round_mod32(zl1.x);
@@ -502,22 +513,24 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
 
     LaneTw<float, R1> tw;
     load_lane_tw(tw, (const C2<float>*)A.t1, (const C2<float>*)A.t2, lane);
-    C2<double> tws[R1];
+    C2<float> tws[R1];
 #pragma unroll
-    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<double>*)A.twist64)[lane * R1 + r];
+    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
+    const C2<double>* untw = (const C2<double>*)A.untw64;
     const float4* key = (const float4*)A.bk;
     constexpr size_t ROWSET = (size_t)KPL * 2 * M;
     const int sb = A.split_bits;
 
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-        step32<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, tws, lane, sb, raw);
+        step32<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, tws, untw, lane, sb,
+                                raw);
     } else {
         const int n = A.n;
         for (int i = 0; i < n; ++i) {
             const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
             if (a == 0) continue;
-            step32<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, tws, lane, sb, nullptr);
+            step32<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, tws, untw, lane, sb, nullptr);
         }
     }
     boot_epilogue<N>(A, g, lane, s_acc);
@@ -808,7 +821,8 @@ __global__ void __launch_bounds__(64) k_fft_test64(double* __restrict__ out, con
 template <int N>
 __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, const void* __restrict__ in, int inverse,
                                                    const C2<float>* __restrict__ t1, const C2<float>* __restrict__ t2,
-                                                   const C2<double>* __restrict__ twist64) {
+                                                   const C2<float>* __restrict__ twist32,
+                                                   const C2<double>* __restrict__ untw64) {
     constexpr int M = N / 2, R1 = M / 64;
     __shared__ C2<float> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
@@ -819,12 +833,9 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
     if (!inverse) {
         const int32_t* p = (const int32_t*)in + g * N;
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) {
-            C2<double> z = cmul(C2<double>{(double)p[lane + 64 * aa], (double)p[lane + 64 * aa + M]},
-                                twist64[lane * R1 + aa]);
-            v[aa] = C2<float>{(float)z.x, (float)z.y};
-        }
-        fft_fwd<R1>(v, tw, s_buf, lane);
+        for (int aa = 0; aa < R1; ++aa)
+            v[aa] = cmul(C2<float>{(float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]}, twist32[lane * R1 + aa]);
+        fft_fwd<R1, float, false, true>(v, tw, s_buf, lane);
         double* o = out + g * N;
 #pragma unroll
         for (int s = 0; s < R1 / 8; ++s)
@@ -843,13 +854,13 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
                 const int k = lane_order_k<R1>(lane, s, k3);
                 v[s * 8 + k3] = C2<float>{(float)F[2 * k], (float)F[2 * k + 1]};
             }
-        fft_inv<R1>(v, tw, s_buf, lane);
+        fft_inv<R1, float, false, true>(v, tw, s_buf, lane);
         double* o = out + g * N;
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
-            C2<double> z = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, twist64[lane * R1 + aa]);
-            o[lane + 64 * aa] = z.x / M;
-            o[lane + 64 * aa + M] = z.y / M;
+            C2<double> z = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, untw64[lane * R1 + aa]);
+            o[lane + 64 * aa] = z.x;
+            o[lane + 64 * aa + M] = z.y;
         }
     }
 }
@@ -897,8 +908,8 @@ static void cis_pi_over_N(long long num, int N, double* re, double* im) {
 
 hipError_t build_twiddles(DevTwiddles& tw, int N) {
     const int M = N / 2, R1 = M / 64;
-    std::vector<double> t1f(2 * 64 * R1), t2(2 * 64 * 8), tws(2 * 64 * R1), wa(2 * R1);
-    std::vector<float> t1p(2 * 64 * R1), t2f(2 * 64 * 8);
+    std::vector<double> t1f(2 * 64 * R1), t2(2 * 64 * 8), untw(2 * 64 * R1), wa(2 * R1);
+    std::vector<float> t1p(2 * 64 * R1), t2f(2 * 64 * 8), tws(2 * 64 * R1);
     for (int L = 0; L < 64; ++L) {
         for (int k1 = 0; k1 < R1; ++k1) {
             double re, im;
@@ -907,7 +918,8 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
             cis_pi_over_N((long long)4 * L * k1, N, &re, &im);           // W_M^{L k1} = w^{4 L k1}
             t1p[2 * (L * R1 + k1)] = (float)re; t1p[2 * (L * R1 + k1) + 1] = (float)im;
             cis_pi_over_N((long long)64 * k1 + L, N, &re, &im);          // w^{64a + L}, a = k1
-            tws[2 * (L * R1 + k1)] = re; tws[2 * (L * R1 + k1) + 1] = im;
+            tws[2 * (L * R1 + k1)] = (float)re; tws[2 * (L * R1 + k1) + 1] = (float)im;
+            untw[2 * (L * R1 + k1)] = re / M; untw[2 * (L * R1 + k1) + 1] = im / M;
         }
         for (int k2 = 0; k2 < 8; ++k2) {
             // W_64^{c k2} = w^{(2N/64) c k2}
@@ -933,13 +945,14 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     if ((e = up(&tw.t2_64, t2.data(), t2.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.t1_plain32, t1p.data(), t1p.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.t2_32, t2f.data(), t2f.size() * 4)) != hipSuccess) return e;
-    if ((e = up(&tw.twist64, tws.data(), tws.size() * 8)) != hipSuccess) return e;
+    if ((e = up(&tw.twist32, tws.data(), tws.size() * 4)) != hipSuccess) return e;
+    if ((e = up(&tw.untw64, untw.data(), untw.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
     return hipSuccess;
 }
 
 void free_twiddles(DevTwiddles& tw) {
-    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist64, tw.wa64};
+    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist32, tw.untw64, tw.wa64};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -962,7 +975,8 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
     A.n = key.p.n;
     A.split_bits = key.split_bits;
     A.wa64 = tw.wa64;
-    A.twist64 = tw.twist64;
+    A.twist32 = tw.twist32;
+    A.untw64 = tw.untw64;
     if (prec == PREC_FP64) {
         A.bk = key.bk64;
         A.t1 = tw.t1_fold64;
@@ -1069,10 +1083,12 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
     } else {
         if (N == 1024)
             hipLaunchKernelGGL((k_fft_test32<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
-                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<double>*)tw.twist64);
+                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<float>*)tw.twist32,
+                               (const C2<double>*)tw.untw64);
         else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test32<2048>), dim3(count), dim3(64), 0, st, out, in, inverse,
-                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<double>*)tw.twist64);
+                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<float>*)tw.twist32,
+                               (const C2<double>*)tw.untw64);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();
