@@ -154,8 +154,11 @@ __device__ __forceinline__ void key_row_to_lds(C2<double>* kbuf, const C2<double
         __builtin_amdgcn_global_load_lds((const void*)(src + pr * 64), (lds_void*)(kbuf + pr * 64), 16, 0, 0);
 }
 
-template <int N, int L, int BG, bool ROT>
-__device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, C2<double>* kbuf, int a,
+// KEY_LDS: key rows come through the LDS buffer kbuf (prefetched one row ahead); otherwise each
+// lane loads its key values from memory right after the forward transform.  HALF: the transform
+// exchanges go through a half-width buffer (real, then imaginary parts).
+template <int N, int L, int BG, bool ROT, bool KEY_LDS = true, bool HALF = false>
+__device__ __forceinline__ void step64(int32_t* s_acc, void* s_buf, C2<double>* kbuf, int a,
                                        const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
                                        const LaneTw<double, N / 128>& tw, const C2<double>* wa, int lane,
                                        double* raw) {
@@ -170,7 +173,7 @@ __device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, C2<dou
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
             const int j = lane + 64 * r;
-            u[r] = (ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF;
+            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
@@ -178,20 +181,28 @@ __device__ __forceinline__ void step64(int32_t* s_acc, C2<double>* s_buf, C2<dou
             C2<double> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
-                v[aa] = C2<double>{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
+                v[aa] = C2<double>{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + R1], lvl)};
                 if (aa) v[aa] = cmul(v[aa], wa[aa]);
             }
-            fft_fwd<R1>(v, tw, s_buf, lane);
+            fft_fwd<R1, double, HALF>(v, tw, s_buf, lane);
+            if constexpr (KEY_LDS) {
 #pragma unroll
-            for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], kbuf[r * 64 + lane]);
+                for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], kbuf[r * 64 + lane]);
 #pragma unroll
-            for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], kbuf[(R1 + r) * 64 + lane]);
-            if (row + 1 < 2 * L) key_row_to_lds<M>(kbuf, kr, row + 1, lane);
-            else key_row_to_lds<M>(kbuf, kr_next, 0, lane);
+                for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], kbuf[(R1 + r) * 64 + lane]);
+                if (row + 1 < 2 * L) key_row_to_lds<M>(kbuf, kr, row + 1, lane);
+                else key_row_to_lds<M>(kbuf, kr_next, 0, lane);
+            } else {
+                const C2<double>* k = kr + (size_t)row * 2 * M + lane;
+#pragma unroll
+                for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64]);
+#pragma unroll
+                for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], k[(R1 + r) * 64]);
+            }
         }
     }
-    fft_inv<R1>(o0, tw, s_buf, lane);
-    fft_inv<R1>(o1, tw, s_buf, lane);
+    fft_inv<R1, double, HALF>(o0, tw, s_buf, lane);
+    fft_inv<R1, double, HALF>(o1, tw, s_buf, lane);
     const double sc = 1.0 / (double)M;
 #pragma unroll
     for (int aa = 0; aa < R1; ++aa) {
@@ -355,7 +366,7 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
             uint32_t u[NC];
             if (work) {
 #pragma unroll
-                for (int r = 0; r < NC; ++r) u[r] = rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF;
+                for (int r = 0; r < NC; ++r) u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
             }
 #pragma unroll
             for (int lvl = 0; lvl < L; ++lvl) {
@@ -364,7 +375,7 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
                 if (work) {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa) {
-                        v[aa] = C2<double>{(double)digit<BG>(u[aa], lvl), (double)digit<BG>(u[aa + R1], lvl)};
+                        v[aa] = C2<double>{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + R1], lvl)};
                         if (aa) v[aa] = cmul(v[aa], wa[aa]);
                     }
                     fft_fwd<R1, double, true>(v, tw, s_x, lane);
@@ -437,14 +448,15 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
             const int j = lane + 64 * r;
-            u[r] = (ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF;
+            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
             C2<float> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
-                v[aa] = cmul(C2<float>{(float)digit<BG>(u[aa], lvl), (float)digit<BG>(u[aa + R1], lvl)}, tws[aa]);
+                v[aa] = cmul(C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)},
+                             tws[aa]);
             fft_fwd<R1, float, false, true>(v, tw, s_buf, lane);
             const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
 #pragma unroll

@@ -341,6 +341,14 @@ __device__ __forceinline__ int digit(uint32_t u_with_offset, int lvl) {
     return (int)((u_with_offset >> (32 - (lvl + 1) * BG)) & ((1u << BG) - 1)) - (1 << (BG - 1));
 }
 
+// The same digit from ux = (x + offset) ^ offset.  The offset is 2^(BG-1) in every BG-bit field,
+// so the xor flips each field's top bit, and field - 2^(BG-1) is then the field read as a signed
+// BG-bit number: one signed bit-field extract instead of an extract and a subtract.
+template <int BG>
+__device__ __forceinline__ int digit_sx(uint32_t ux, int lvl) {
+    return ((int32_t)(ux << (lvl * BG))) >> (32 - BG);
+}
+
 template <int L, int BG>
 __host__ __device__ constexpr uint32_t decomp_offset() {
     uint32_t off = 0;
