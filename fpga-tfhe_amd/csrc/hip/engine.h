@@ -22,6 +22,11 @@ struct DevTwiddles {
     void* twist32 = nullptr;     // [64][R1] C2<float>:  w^{64a + L}      (split path, forward twist)
     void* untw64 = nullptr;      // [64][R1] C2<double>: w^{64a + L} / M  (split path, untwist)
     void* wa64 = nullptr;        // [R1]     C2<double>: w^{64a}
+    // N = 2048, one transform over two wavefronts (wavefront v holds the frequencies of parity v):
+    void* t1w64 = nullptr;       // [2][64][8] C2<double>: w^{L (1 + 4v + 8 k1)}
+    void* t1w32 = nullptr;       // [2][64][8] C2<float>
+    void* rw64 = nullptr;        // [2][8] C2<double>: w^{64 a (1 + 4v)}
+    void* rw32 = nullptr;        // [2][8] C2<float>
 };
 
 // One gate segment of a table-driven bootstrap launch (mode 3): jobs [s*seg_jobs, (s+1)*seg_jobs)

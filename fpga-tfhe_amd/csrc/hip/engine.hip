@@ -10,6 +10,7 @@
 //   k_ksk_relayout       keyswitch key -> [kN][t][base-1][NP] without the unused h=0 rows.
 //   k_extract, k_rotdec, k_fft_test, k_decrypt  stage kernels for parity tests and audits.
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -36,6 +37,8 @@ struct BootKArgs {
     const void* twist32;
     const void* untw64;
     const void* wa64;
+    const void* t2w_rw64;   // N = 2048 two-wavefront register twists [2][8] (FP64)
+    const void* t2w_rw32;   // and FP32
     int n;
     int split_bits;
 };
@@ -946,6 +949,28 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
         cis_pi_over_N((long long)64 * a, N, &re, &im);
         wa[2 * a] = re; wa[2 * a + 1] = im;
     }
+    // two-wavefront form (N = 2048): pass-1 twiddles of the 512-point transform of wavefront v with
+    // its lane twist w^{(1+4v) L} folded in, and its register twist constants w^{64 a (1+4v)}
+    std::vector<double> t1w(2 * 2 * 64 * 8), rw(2 * 2 * 8);
+    std::vector<float> t1w32(t1w.size()), rw32(rw.size());
+    if (N == 2048) {
+        for (int v = 0; v < 2; ++v) {
+            for (int L = 0; L < 64; ++L)
+                for (int k1 = 0; k1 < 8; ++k1) {
+                    double re, im;
+                    cis_pi_over_N((long long)L * (1 + 4 * v + 8 * k1), N, &re, &im);
+                    const size_t o = 2 * ((size_t)(v * 64 + L) * 8 + k1);
+                    t1w[o] = re; t1w[o + 1] = im;
+                    t1w32[o] = (float)re; t1w32[o + 1] = (float)im;
+                }
+            for (int a = 0; a < 8; ++a) {
+                double re, im;
+                cis_pi_over_N((long long)64 * a * (1 + 4 * v), N, &re, &im);
+                rw[2 * (v * 8 + a)] = re; rw[2 * (v * 8 + a) + 1] = im;
+                rw32[2 * (v * 8 + a)] = (float)re; rw32[2 * (v * 8 + a) + 1] = (float)im;
+            }
+        }
+    }
     hipError_t e;
     auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
         hipError_t r = hipMalloc(dst, bytes);
@@ -960,11 +985,18 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     if ((e = up(&tw.twist32, tws.data(), tws.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.untw64, untw.data(), untw.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
+    if (N == 2048) {
+        if ((e = up(&tw.t1w64, t1w.data(), t1w.size() * 8)) != hipSuccess) return e;
+        if ((e = up(&tw.t1w32, t1w32.data(), t1w32.size() * 4)) != hipSuccess) return e;
+        if ((e = up(&tw.rw64, rw.data(), rw.size() * 8)) != hipSuccess) return e;
+        if ((e = up(&tw.rw32, rw32.data(), rw32.size() * 4)) != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
 void free_twiddles(DevTwiddles& tw) {
-    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist32, tw.untw64, tw.wa64};
+    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist32, tw.untw64, tw.wa64,
+                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -973,6 +1005,418 @@ void free_twiddles(DevTwiddles& tw) {
 bool params_supported(const DevParams& p) {
     const bool ring = (p.N == 1024 && p.l == 3 && p.Bgbit == 7) || (p.N == 2048 && p.l == 3 && p.Bgbit == 8);
     return ring && p.n > 0 && p.n <= kMaxLweN && p.ks_basebit == 2 && (p.ks_t == 8 || p.ks_t == 10);
+}
+
+// ---------------------------------------------------------------------------------------------
+// N = 2048: each folded transform (M = 1024 points) is split over the two wavefronts of a
+// bootstrap, so that every wavefront keeps the 8-points-per-lane register shape of N = 1024.
+// With z_j = P_j w^j (P_j = p_j + i p_{j+1024}, w = e^{i pi/2048}) and the split j = j' + 512h of
+// the transform sum, the frequencies of parity v are a 512-point transform of
+//   y_v[j'] = w^{(1+4v) j'} (P_{j'} + (-1)^v e^{i pi/4} P_{j'+512}),        j' < 512,
+// because w^512 = e^{i pi/4} and the odd half carries W_1024^{j'} = w^{4j'}.  Wavefront v builds
+// y_v from all 2048 coefficients (no exchange on the way in) and runs the 512-point transform with
+// the lane part of its twist folded into the pass-1 twiddles; its result is F_{2m+v}, m in lane
+// order.  The key is stored in the same [parity][register][lane] order.  On the way back each
+// wavefront inverts its own half; with E = wavefront 0's and O = wavefront 1's output after the
+// register untwist,
+//   (p_{j'}, p_{j'+1024})       = (E + O) / 1024                     (wavefront 0 keeps these)
+//   (p_{j'+512}, p_{j'+1536})   = (E - O) e^{-i pi/4} / 1024          (wavefront 1 keeps these)
+// which needs one exchange of the inverse outputs through LDS.
+// ---------------------------------------------------------------------------------------------
+constexpr int W2_N = 2048, W2_M = 1024, W2_H = 512;
+
+template <typename T>
+__device__ __forceinline__ C2<T> w2_c() {
+    const T h = (T)0.70710678118654752440;   // e^{i pi/4}
+    return C2<T>{h, h};
+}
+
+// y_v[j'] before its lane twist: register twist w^{64a(1+4v)} (1 for a = 0) times P0 +- c P1
+template <typename T>
+__device__ __forceinline__ C2<T> w2_prep(C2<T> p0, C2<T> p1, C2<T> rw, int wave, bool first) {
+    const C2<T> t = cmul(p1, w2_c<T>());
+    const C2<T> y = wave ? C2<T>{p0.x - t.x, p0.y - t.y} : C2<T>{p0.x + t.x, p0.y + t.y};
+    return first ? y : cmul(y, rw);
+}
+
+// the pair this wavefront keeps from its own inverse output X (register-twisted, unscaled) and the
+// partner's output Y, both already untwisted: wavefront 0 -> (E + O)/1024, 1 -> (E - O) c^* /1024
+__device__ __forceinline__ C2<double> w2_pair(C2<double> own, C2<double> other, int wave) {
+    const double s = 1.0 / (double)W2_M;
+    if (!wave) return C2<double>{(own.x + other.x) * s, (own.y + other.y) * s};
+    const C2<double> d{(other.x - own.x) * s, (other.y - own.y) * s};   // E - O, E is the partner's
+    return cmulc(d, w2_c<double>());
+}
+
+// coefficient index (within one polynomial) of the real / imaginary part that wavefront `wave`
+// keeps for register a
+__device__ __forceinline__ int w2_coef(int wave, int lane, int a, int imag) {
+    return lane + 64 * a + 512 * wave + 1024 * imag;
+}
+
+// Touches every 128-byte line of this wavefront's half of a key row (2 polynomials x 512 entries of
+// 16 bytes): one dword per lane per polynomial, folded into `sink` so the loads stay live.  Issued
+// after a row's key loads, it brings the next row into L2 while the next transform runs; that row's
+// own loads then hit L2 instead of stalling on the ~2% of lines that would come from memory.
+__device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, float& sink) {
+    const float* p = (const float*)row_half;
+    sink += p[lane * 32] + p[W2_M * 4 + lane * 32];
+}
+
+// Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.
+template <int L, int BG, bool ROT>
+__device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, const C2<float>* sx_other, int a,
+                                         const float4* __restrict__ kr, const float4* __restrict__ kr_next,
+                                         const LaneTw<float, 8>& tw,
+                                         const C2<float>* __restrict__ rw32, const C2<double>* __restrict__ rw_own,
+                                         const C2<double>* __restrict__ rw_other, int wave, int lane, int split_bits,
+                                         double* raw) {
+    constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        h0[r] = C2<float>{0.f, 0.f}; l0[r] = C2<float>{0.f, 0.f};
+        h1[r] = C2<float>{0.f, 0.f}; l1[r] = C2<float>{0.f, 0.f};
+    }
+    float sink = 0.f;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t u[NC];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const int j = lane + 64 * r;
+            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
+        }
+#pragma unroll
+        for (int lvl = 0; lvl < L; ++lvl) {
+            const int row = q * L + lvl;
+            C2<float> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const C2<float> p0{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl)};
+                const C2<float> p1{(float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl)};
+                v[aa] = w2_prep(p0, p1, rw32[aa], wave, aa == 0);
+            }
+            fft_fwd<R1>(v, tw, sx_own, lane);
+            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const float4 k0 = k[r * 64];
+                cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
+                cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
+            }
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const float4 k1 = k[M + r * 64];
+                cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
+                cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
+            }
+            // issued after this row's key loads (vmcnt drains in issue order), so it overlaps the
+            // next transform and gates nothing but the next row's loads
+            __builtin_amdgcn_sched_barrier(0);
+            w2_warm_row(row + 1 < 2 * L ? kr + (size_t)((row + 1) * 2) * M + wave * W2_H : kr_next + wave * W2_H, lane,
+                        sink);
+        }
+    }
+    asm volatile("" ::"v"(sink));
+    fft_inv<R1>(h0, tw, sx_own, lane);
+    fft_inv<R1>(l0, tw, sx_own, lane);
+    fft_inv<R1>(h1, tw, sx_own, lane);
+    fft_inv<R1>(l1, tw, sx_own, lane);
+    const double hs = ldexp(1.0, split_bits);
+    // one output polynomial at a time: exchange hi, combine and round it, then lo
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t rh[2 * R1];
+        C2<double> ph[R1];
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+            const C2<float>* X = part ? (q ? l1 : l0) : (q ? h1 : h0);
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa];
+            __syncthreads();
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const C2<float> y = sx_other[aa * 64 + lane];
+                const C2<double> own = cmulc(C2<double>{(double)X[aa].x, (double)X[aa].y}, rw_own[aa]);
+                const C2<double> oth = cmulc(C2<double>{(double)y.x, (double)y.y}, rw_other[aa]);
+                const C2<double> z = w2_pair(own, oth, wave);
+                if (!part) {
+                    rh[2 * aa] = round_mod32(z.x);
+                    rh[2 * aa + 1] = round_mod32(z.y);
+                    ph[aa] = z;
+                } else {
+                    const uint32_t c0 = (rh[2 * aa] << split_bits) + round_mod32(z.x);
+                    const uint32_t c1 = (rh[2 * aa + 1] << split_bits) + round_mod32(z.y);
+                    const int j0 = q * N + w2_coef(wave, lane, aa, 0), j1 = q * N + w2_coef(wave, lane, aa, 1);
+                    if (raw) { raw[j0] = ph[aa].x * hs + z.x; raw[j1] = ph[aa].y * hs + z.y; }
+                    if (ROT) {
+                        s_acc[j0] = (int32_t)((uint32_t)s_acc[j0] + c0);
+                        s_acc[j1] = (int32_t)((uint32_t)s_acc[j1] + c1);
+                    } else {
+                        s_acc[j0] = (int32_t)c0;
+                        s_acc[j1] = (int32_t)c1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// FP64 step at N = 2048, one wavefront's share; exchanges through half-width buffers.
+template <int L, int BG, bool ROT>
+__device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const double* sx_other, int a,
+                                         const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
+                                         const LaneTw<double, 8>& tw,
+                                         const C2<double>* __restrict__ rw_own, const C2<double>* __restrict__ rw_other,
+                                         int wave, int lane, double* raw) {
+    constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    C2<double> o0[R1], o1[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
+    float sink = 0.f;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t u[NC];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const int j = lane + 64 * r;
+            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
+        }
+#pragma unroll
+        for (int lvl = 0; lvl < L; ++lvl) {
+            const int row = q * L + lvl;
+            C2<double> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const C2<double> p0{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + 16], lvl)};
+                const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
+                v[aa] = w2_prep(p0, p1, rw_own[aa], wave, aa == 0);
+            }
+            fft_fwd<R1, double, true>(v, tw, sx_own, lane);
+            const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
+#pragma unroll
+            for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64]);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], k[M + r * 64]);
+            __builtin_amdgcn_sched_barrier(0);
+            w2_warm_row(row + 1 < 2 * L ? kr + (size_t)((row + 1) * 2) * M + wave * W2_H : kr_next + wave * W2_H, lane,
+                        sink);
+        }
+    }
+    asm volatile("" ::"v"(sink));
+    fft_inv<R1, double, true>(o0, tw, sx_own, lane);
+    fft_inv<R1, double, true>(o1, tw, sx_own, lane);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const C2<double>* X = q ? o1 : o0;
+        double yx[R1], yy[R1];
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].x;
+        __syncthreads();
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) yx[aa] = sx_other[aa * 64 + lane];
+        __syncthreads();
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].y;
+        __syncthreads();
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) yy[aa] = sx_other[aa * 64 + lane];
+        __syncthreads();
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const C2<double> own = aa ? cmulc(X[aa], rw_own[aa]) : X[aa];
+            const C2<double> oth = aa ? cmulc(C2<double>{yx[aa], yy[aa]}, rw_other[aa]) : C2<double>{yx[aa], yy[aa]};
+            const C2<double> z = w2_pair(own, oth, wave);
+            const int j0 = q * N + w2_coef(wave, lane, aa, 0), j1 = q * N + w2_coef(wave, lane, aa, 1);
+            if (raw) { raw[j0] = z.x; raw[j1] = z.y; }
+            const uint32_t c0 = round_mod32(z.x), c1 = round_mod32(z.y);
+            if (ROT) {
+                s_acc[j0] = (int32_t)((uint32_t)s_acc[j0] + c0);
+                s_acc[j1] = (int32_t)((uint32_t)s_acc[j1] + c1);
+            } else {
+                s_acc[j0] = (int32_t)c0;
+                s_acc[j1] = (int32_t)c1;
+            }
+        }
+    }
+    // both wavefronts read every accumulator coefficient at the start of the next step
+    __syncthreads();
+}
+
+// Bootstrap at N = 2048, two wavefronts per bootstrap.  P = 0: FP64, P = 1: split FP32.
+template <int L, int BG, int P>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(P ? 2 : 1, P ? 2 : 1)))
+k_boot2w(BootKArgs A) {
+    constexpr int N = W2_N, M = W2_M, KPL = 2 * L;
+    constexpr int XB = xbuf_elems<W2_H>();
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ double s_xd[2][XB];   // per wavefront: C2<float>[XB] (8 bytes each) or double[XB]
+    __shared__ int16_t s_bara[kMaxLweN];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = blockIdx.x;
+    if (wave == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    __syncthreads();
+    const C2<double>* rw64 = (const C2<double>*)A.t2w_rw64;
+    const C2<double>* rw_own = rw64 + wave * 8;
+    const C2<double>* rw_other = rw64 + (1 - wave) * 8;
+    constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+    if constexpr (P == 1) {
+        LaneTw<float, 8> tw;
+        load_lane_tw(tw, (const C2<float>*)A.t1 + wave * 64 * 8, (const C2<float>*)A.t2, lane);
+        const C2<float>* rw32 = (const C2<float>*)A.t2w_rw32 + wave * 8;
+        C2<float>* sx_own = (C2<float>*)s_xd[wave];
+        const C2<float>* sx_other = (const C2<float>*)s_xd[1 - wave];
+        const float4* key = (const float4*)A.bk;
+        if (A.L.mode == 2) {
+            double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+            const float4* kr = key + (size_t)A.L.row_set[g] * ROWSET;
+            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, kr, kr, tw, rw32, rw_own, rw_other, wave, lane,
+                                   A.split_bits, raw);
+        } else {
+            for (int i = 0; i < A.n; ++i) {
+                const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+                if (a == 0) continue;   // tfhe_blindRotate_FFT.cpp:114; a is the same in both wavefronts
+                step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
+                                      key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw32, rw_own, rw_other,
+                                      wave, lane, A.split_bits, nullptr);
+            }
+        }
+    } else {
+        LaneTw<double, 8> tw;
+        load_lane_tw(tw, (const C2<double>*)A.t1 + wave * 64 * 8, (const C2<double>*)A.t2, lane);
+        double* sx_own = s_xd[wave];
+        const double* sx_other = s_xd[1 - wave];
+        const C2<double>* key = (const C2<double>*)A.bk;
+        if (A.L.mode == 2) {
+            double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+            const C2<double>* kr = key + (size_t)A.L.row_set[g] * ROWSET;
+            step2w64<L, BG, false>(s_acc, sx_own, sx_other, 0, kr, kr, tw, rw_own, rw_other, wave, lane, raw);
+        } else {
+            for (int i = 0; i < A.n; ++i) {
+                const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+                if (a == 0) continue;
+                step2w64<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
+                                      key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw_own, rw_other, wave,
+                                      lane, nullptr);
+            }
+        }
+    }
+    __syncthreads();
+    if (wave == 0) boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// Key preparation at N = 2048: one two-wavefront workgroup per key polynomial, the same prep and
+// 512-point transform as the bootstrap (FP64), stored as [parity][register][lane].
+__global__ void __launch_bounds__(128) k_keyprep2w(const int32_t* __restrict__ bk, C2<double>* __restrict__ out64,
+                                                   float4* __restrict__ out32, int split_bits,
+                                                   const C2<double>* __restrict__ t1w, const C2<double>* __restrict__ t2,
+                                                   const C2<double>* __restrict__ rw) {
+    constexpr int N = W2_N, M = W2_M, R1 = 8;
+    __shared__ C2<double> s_buf[2][xbuf_elems<W2_H>()];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * N;
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, t1w + wave * 64 * 8, t2, lane);
+    const C2<double>* rwv = rw + wave * 8;
+    const size_t obase = poly * M + wave * W2_H;
+    C2<double> v[R1];
+    if (out64) {
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const int j = lane + 64 * aa;
+            v[aa] = w2_prep(C2<double>{(double)p[j], (double)p[j + 1024]}, C2<double>{(double)p[j + 512], (double)p[j + 1536]},
+                            rwv[aa], wave, aa == 0);
+        }
+        fft_fwd<R1>(v, tw, s_buf[wave], lane);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) out64[obase + r * 64 + lane] = v[r];
+    }
+    if (out32) {
+        C2<double> h[R1];
+        const int64_t half = (int64_t)1 << (split_bits - 1);
+        auto hi = [&](int64_t x) { return (x + half) >> split_bits; };
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const int j = lane + 64 * aa;
+            const int64_t x0 = p[j], x1 = p[j + 1024], x2 = p[j + 512], x3 = p[j + 1536];
+            const int64_t h0 = hi(x0), h1 = hi(x1), h2 = hi(x2), h3 = hi(x3);
+            h[aa] = w2_prep(C2<double>{(double)h0, (double)h1}, C2<double>{(double)h2, (double)h3}, rwv[aa], wave, aa == 0);
+            v[aa] = w2_prep(C2<double>{(double)(x0 - (h0 << split_bits)), (double)(x1 - (h1 << split_bits))},
+                            C2<double>{(double)(x2 - (h2 << split_bits)), (double)(x3 - (h3 << split_bits))}, rwv[aa],
+                            wave, aa == 0);
+        }
+        fft_fwd<R1>(h, tw, s_buf[wave], lane);
+        fft_fwd<R1>(v, tw, s_buf[wave], lane);
+#pragma unroll
+        for (int r = 0; r < R1; ++r)
+            out32[obase + r * 64 + lane] = make_float4((float)h[r].x, (float)h[r].y, (float)v[r].x, (float)v[r].y);
+    }
+}
+
+// Transform test kernel at N = 2048 (two wavefronts): forward int32[N] -> F_k natural order, or
+// inverse F natural -> polynomial (FP64 values).
+template <int P>
+__global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, const void* __restrict__ in, int inverse,
+                                                    const void* __restrict__ t1w, const void* __restrict__ t2,
+                                                    const C2<float>* __restrict__ rw32,
+                                                    const C2<double>* __restrict__ rw64) {
+    using T = typename std::conditional<P == 1, float, double>::type;
+    constexpr int N = W2_N, R1 = 8;
+    __shared__ C2<double> s_buf[2][xbuf_elems<W2_H>()];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const size_t g = blockIdx.x;
+    LaneTw<T, R1> tw;
+    load_lane_tw(tw, (const C2<T>*)t1w + wave * 64 * 8, (const C2<T>*)t2, lane);
+    C2<T> v[R1];
+    double* o = out + g * N;
+    if (!inverse) {
+        const int32_t* p = (const int32_t*)in + g * N;
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const int j = lane + 64 * aa;
+            C2<T> rwa;
+            if constexpr (P == 1) rwa = rw32[wave * 8 + aa];
+            else rwa = rw64[wave * 8 + aa];
+            v[aa] = w2_prep(C2<T>{(T)p[j], (T)p[j + 1024]}, C2<T>{(T)p[j + 512], (T)p[j + 1536]}, rwa, wave, aa == 0);
+        }
+        fft_fwd<R1>(v, tw, s_buf[wave], lane);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int k = 2 * (lane + 64 * r) + wave;
+            o[2 * k] = v[r].x;
+            o[2 * k + 1] = v[r].y;
+        }
+    } else {
+        const double* F = (const double*)in + g * N;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int k = 2 * (lane + 64 * r) + wave;
+            v[r] = C2<T>{(T)F[2 * k], (T)F[2 * k + 1]};
+        }
+        fft_inv<R1>(v, tw, s_buf[wave], lane);
+        __syncthreads();
+        C2<double>* xo = (C2<double>*)s_buf[wave];
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) xo[aa * 64 + lane] = C2<double>{(double)v[aa].x, (double)v[aa].y};
+        __syncthreads();
+        const C2<double>* xp = (const C2<double>*)s_buf[1 - wave];
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            const C2<double> own = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, rw64[wave * 8 + aa]);
+            const C2<double> oth = cmulc(xp[aa * 64 + lane], rw64[(1 - wave) * 8 + aa]);
+            const C2<double> z = w2_pair(own, oth, wave);
+            o[w2_coef(wave, lane, aa, 0)] = z.x;
+            o[w2_coef(wave, lane, aa, 1)] = z.y;
+        }
+    }
 }
 
 // FP64 bootstrap form: workgroups sharing key rows (default) or one bootstrap per workgroup
@@ -1016,7 +1460,27 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
     if (Lc.count <= 0) return hipSuccess;
     if ((prec == PREC_FP64 && !key.bk64) || (prec == PREC_SPLIT32 && !key.bk32)) return hipErrorInvalidValue;
     if (key.p.N == 1024) return launch_boot_t<1024, 3, 7>(key, tw, prec, Lc, st);
-    if (key.p.N == 2048) return launch_boot_t<2048, 3, 8>(key, tw, prec, Lc, st);
+    if (key.p.N == 2048) {
+        // two wavefronts per bootstrap (k_boot2w); every mode is supported
+        BootKArgs A;
+        A.L = Lc;
+        A.n = key.p.n;
+        A.split_bits = key.split_bits;
+        A.t2w_rw64 = tw.rw64;
+        A.t2w_rw32 = tw.rw32;
+        if (prec == PREC_FP64) {
+            A.bk = key.bk64;
+            A.t1 = tw.t1w64;
+            A.t2 = tw.t2_64;
+            hipLaunchKernelGGL((k_boot2w<3, 8, 0>), dim3(Lc.count), dim3(128), 0, st, A);
+        } else {
+            A.bk = key.bk32;
+            A.t1 = tw.t1w32;
+            A.t2 = tw.t2_32;
+            hipLaunchKernelGGL((k_boot2w<3, 8, 1>), dim3(Lc.count), dim3(128), 0, st, A);
+        }
+        return hipGetLastError();
+    }
     return hipErrorInvalidValue;
 }
 
@@ -1051,9 +1515,9 @@ hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32
                            split_bits, (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64,
                            (const C2<double>*)tw.wa64);
     else if (p.N == 2048)
-        hipLaunchKernelGGL((k_keyprep<2048>), dim3(npoly), dim3(64), 0, st, d_bk, (C2<double>*)bk64, (float4*)bk32,
-                           split_bits, (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64,
-                           (const C2<double>*)tw.wa64);
+        hipLaunchKernelGGL(k_keyprep2w, dim3(npoly), dim3(128), 0, st, d_bk, (C2<double>*)bk64, (float4*)bk32,
+                           split_bits, (const C2<double>*)tw.t1w64, (const C2<double>*)tw.t2_64,
+                           (const C2<double>*)tw.rw64);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1089,8 +1553,8 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
             hipLaunchKernelGGL((k_fft_test64<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
                                (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
         else if (N == 2048)
-            hipLaunchKernelGGL((k_fft_test64<2048>), dim3(count), dim3(64), 0, st, out, in, inverse,
-                               (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
+            hipLaunchKernelGGL((k_fft_test2w<0>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w64, tw.t2_64,
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64);
         else return hipErrorInvalidValue;
     } else {
         if (N == 1024)
@@ -1098,9 +1562,8 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<float>*)tw.twist32,
                                (const C2<double>*)tw.untw64);
         else if (N == 2048)
-            hipLaunchKernelGGL((k_fft_test32<2048>), dim3(count), dim3(64), 0, st, out, in, inverse,
-                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<float>*)tw.twist32,
-                               (const C2<double>*)tw.untw64);
+            hipLaunchKernelGGL((k_fft_test2w<1>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w32, tw.t2_32,
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();
