@@ -1070,7 +1070,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
                                          const LaneTw<float, 8>& tw,
                                          const C2<float>* __restrict__ rw32, const C2<double>* __restrict__ rw_own,
                                          const C2<double>* __restrict__ rw_other, int wave, int lane, int split_bits,
-                                         double* raw) {
+                                         double* raw, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
@@ -1080,6 +1080,9 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
         h1[r] = C2<float>{0.f, 0.f}; l1[r] = C2<float>{0.f, 0.f};
     }
     float sink = 0.f;
+    // an idle bootstrap (zero rotation, or past the batch end) skips the arithmetic but still takes
+    // part in every workgroup barrier below
+    if (work) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
@@ -1124,6 +1127,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
     fft_inv<R1>(l0, tw, sx_own, lane);
     fft_inv<R1>(h1, tw, sx_own, lane);
     fft_inv<R1>(l1, tw, sx_own, lane);
+    }
     const double hs = ldexp(1.0, split_bits);
     // one output polynomial at a time: exchange hi, combine and round it, then lo
 #pragma unroll
@@ -1133,11 +1137,14 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
             const C2<float>* X = part ? (q ? l1 : l0) : (q ? h1 : h0);
+            if (work) {
 #pragma unroll
-            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa];
+                for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa];
+            }
             __syncthreads();
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
+                if (!work) break;
                 const C2<float> y = sx_other[aa * 64 + lane];
                 const C2<double> own = cmulc(C2<double>{(double)X[aa].x, (double)X[aa].y}, rw_own[aa]);
                 const C2<double> oth = cmulc(C2<double>{(double)y.x, (double)y.y}, rw_other[aa]);
@@ -1171,13 +1178,14 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
                                          const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
                                          const LaneTw<double, 8>& tw,
                                          const C2<double>* __restrict__ rw_own, const C2<double>* __restrict__ rw_other,
-                                         int wave, int lane, double* raw) {
+                                         int wave, int lane, double* raw, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<double> o0[R1], o1[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
     float sink = 0.f;
+    if (work) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
@@ -1210,24 +1218,34 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
     asm volatile("" ::"v"(sink));
     fft_inv<R1, double, true>(o0, tw, sx_own, lane);
     fft_inv<R1, double, true>(o1, tw, sx_own, lane);
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const C2<double>* X = q ? o1 : o0;
         double yx[R1], yy[R1];
+        if (work) {
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].x;
+            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].x;
+        }
         __syncthreads();
+        if (work) {
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) yx[aa] = sx_other[aa * 64 + lane];
+            for (int aa = 0; aa < R1; ++aa) yx[aa] = sx_other[aa * 64 + lane];
+        }
         __syncthreads();
+        if (work) {
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].y;
+            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].y;
+        }
         __syncthreads();
+        if (work) {
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) yy[aa] = sx_other[aa * 64 + lane];
+            for (int aa = 0; aa < R1; ++aa) yy[aa] = sx_other[aa * 64 + lane];
+        }
         __syncthreads();
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
+            if (!work) break;
             const C2<double> own = aa ? cmulc(X[aa], rw_own[aa]) : X[aa];
             const C2<double> oth = aa ? cmulc(C2<double>{yx[aa], yy[aa]}, rw_other[aa]) : C2<double>{yx[aa], yy[aa]};
             const C2<double> z = w2_pair(own, oth, wave);
@@ -1247,67 +1265,73 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
     __syncthreads();
 }
 
-// Bootstrap at N = 2048, two wavefronts per bootstrap.  P = 0: FP64, P = 1: split FP32.
-template <int L, int BG, int P>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(P ? 2 : 1, P ? 2 : 1)))
+// Bootstrap at N = 2048, two wavefronts per bootstrap, GB bootstraps per workgroup (their key reads
+// of a row come close together and share the CU's L1; the exchange barriers keep them in step).
+// P = 0: FP64, P = 1: split FP32.
+template <int L, int BG, int P, int GB>
+__global__ void __launch_bounds__(128 * GB) __attribute__((amdgpu_waves_per_eu(P ? 2 : 1, P ? 2 : 1)))
 k_boot2w(BootKArgs A) {
     constexpr int N = W2_N, M = W2_M, KPL = 2 * L;
     constexpr int XB = xbuf_elems<W2_H>();
-    __shared__ int32_t s_acc[2 * N];
-    __shared__ double s_xd[2][XB];   // per wavefront: C2<float>[XB] (8 bytes each) or double[XB]
-    __shared__ int16_t s_bara[kMaxLweN];
+    __shared__ int32_t s_acc_all[GB][2 * N];
+    __shared__ double s_xd[GB][2][XB];   // per wavefront: C2<float>[XB] (8 bytes each) or double[XB]
+    __shared__ int16_t s_bara_all[GB][kMaxLweN];
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int g = blockIdx.x;
-    if (wave == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int b = wid >> 1, wave = wid & 1;
+    const int g = blockIdx.x * GB + b;
+    const bool active = g < A.L.count;
+    int32_t* s_acc = s_acc_all[b];
+    int16_t* s_bara = s_bara_all[b];
+    if (active && wave == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
     __syncthreads();
     const C2<double>* rw64 = (const C2<double>*)A.t2w_rw64;
     const C2<double>* rw_own = rw64 + wave * 8;
     const C2<double>* rw_other = rw64 + (1 - wave) * 8;
     constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+    const size_t rs = (A.L.mode == 2 && active) ? (size_t)A.L.row_set[g] * ROWSET : 0;
     if constexpr (P == 1) {
         LaneTw<float, 8> tw;
         load_lane_tw(tw, (const C2<float>*)A.t1 + wave * 64 * 8, (const C2<float>*)A.t2, lane);
         const C2<float>* rw32 = (const C2<float>*)A.t2w_rw32 + wave * 8;
-        C2<float>* sx_own = (C2<float>*)s_xd[wave];
-        const C2<float>* sx_other = (const C2<float>*)s_xd[1 - wave];
+        C2<float>* sx_own = (C2<float>*)s_xd[b][wave];
+        const C2<float>* sx_other = (const C2<float>*)s_xd[b][1 - wave];
         const float4* key = (const float4*)A.bk;
         if (A.L.mode == 2) {
-            double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-            const float4* kr = key + (size_t)A.L.row_set[g] * ROWSET;
-            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, kr, kr, tw, rw32, rw_own, rw_other, wave, lane,
-                                   A.split_bits, raw);
+            double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw32, rw_own, rw_other, wave,
+                                   lane, A.split_bits, raw, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
-                const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
-                if (a == 0) continue;   // tfhe_blindRotate_FFT.cpp:114; a is the same in both wavefronts
+                // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); a is the same in both
+                // wavefronts of a bootstrap, and an idle bootstrap still meets every barrier
+                const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
                 step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
                                       key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw32, rw_own, rw_other,
-                                      wave, lane, A.split_bits, nullptr);
+                                      wave, lane, A.split_bits, nullptr, a != 0);
             }
         }
     } else {
         LaneTw<double, 8> tw;
         load_lane_tw(tw, (const C2<double>*)A.t1 + wave * 64 * 8, (const C2<double>*)A.t2, lane);
-        double* sx_own = s_xd[wave];
-        const double* sx_other = s_xd[1 - wave];
+        double* sx_own = s_xd[b][wave];
+        const double* sx_other = s_xd[b][1 - wave];
         const C2<double>* key = (const C2<double>*)A.bk;
         if (A.L.mode == 2) {
-            double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-            const C2<double>* kr = key + (size_t)A.L.row_set[g] * ROWSET;
-            step2w64<L, BG, false>(s_acc, sx_own, sx_other, 0, kr, kr, tw, rw_own, rw_other, wave, lane, raw);
+            double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+            step2w64<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw_own, rw_other, wave, lane,
+                                   raw, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
-                const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
-                if (a == 0) continue;
+                const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
                 step2w64<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
                                       key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw_own, rw_other, wave,
-                                      lane, nullptr);
+                                      lane, nullptr, a != 0);
             }
         }
     }
     __syncthreads();
-    if (wave == 0) boot_epilogue<N>(A, g, lane, s_acc);
+    if (active && wave == 0) boot_epilogue<N>(A, g, lane, s_acc);
 }
 
 // Key preparation at N = 2048: one two-wavefront workgroup per key polynomial, the same prep and
@@ -1472,12 +1496,14 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
             A.bk = key.bk64;
             A.t1 = tw.t1w64;
             A.t2 = tw.t2_64;
-            hipLaunchKernelGGL((k_boot2w<3, 8, 0>), dim3(Lc.count), dim3(128), 0, st, A);
+            constexpr int GB = 2;   // 4 wavefronts: one workgroup per CU at one wavefront per SIMD
+            hipLaunchKernelGGL((k_boot2w<3, 8, 0, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         } else {
             A.bk = key.bk32;
             A.t1 = tw.t1w32;
             A.t2 = tw.t2_32;
-            hipLaunchKernelGGL((k_boot2w<3, 8, 1>), dim3(Lc.count), dim3(128), 0, st, A);
+            constexpr int GB = 4;   // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD
+            hipLaunchKernelGGL((k_boot2w<3, 8, 1, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         }
         return hipGetLastError();
     }
