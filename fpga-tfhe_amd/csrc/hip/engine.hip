@@ -1176,11 +1176,17 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
 template <int L, int BG, bool ROT>
 __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const double* sx_other, int a,
                                          const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
-                                         const LaneTw<double, 8>& tw,
+                                         const LaneTwS<double, 8>& tw0,
                                          const C2<double>* __restrict__ rw_own, const C2<double>* __restrict__ rw_other,
                                          int wave, int lane, double* raw, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
+    // an offset the compiler cannot see through keeps the pass-2 twiddle reads at their uses (held
+    // across steps they would cost 28 registers)
+    int t2o = 0;
+    asm volatile("" : "+v"(t2o));
+    LaneTwS<double, 8> tw = tw0;
+    tw.t2 = tw0.t2 + t2o;
     C2<double> o0[R1], o1[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
@@ -1269,13 +1275,15 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
 // of a row come close together and share the CU's L1; the exchange barriers keep them in step).
 // P = 0: FP64, P = 1: split FP32.
 template <int L, int BG, int P, int GB>
-__global__ void __launch_bounds__(128 * GB) __attribute__((amdgpu_waves_per_eu(P ? 2 : 1, P ? 2 : 1)))
+__global__ void __launch_bounds__(128 * GB) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_boot2w(BootKArgs A) {
     constexpr int N = W2_N, M = W2_M, KPL = 2 * L;
     constexpr int XB = xbuf_elems<W2_H>();
+    constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
     __shared__ int32_t s_acc_all[GB][2 * N];
     __shared__ double s_xd[GB][2][XB];   // per wavefront: C2<float>[XB] (8 bytes each) or double[XB]
     __shared__ int16_t s_bara_all[GB][kMaxLweN];
+    __shared__ C2<double> s_t2[P ? 1 : 8 * T2S];
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int b = wid >> 1, wave = wid & 1;
@@ -1284,6 +1292,10 @@ k_boot2w(BootKArgs A) {
     int32_t* s_acc = s_acc_all[b];
     int16_t* s_bara = s_bara_all[b];
     if (active && wave == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if constexpr (P == 0) {
+        // lane c of the pass-2 table holds W_64^{c k2}, k2 < 8
+        if (threadIdx.x < 64) s_t2[(threadIdx.x >> 3) * T2S + (threadIdx.x & 7)] = ((const C2<double>*)A.t2)[threadIdx.x];
+    }
     __syncthreads();
     const C2<double>* rw64 = (const C2<double>*)A.t2w_rw64;
     const C2<double>* rw_own = rw64 + wave * 8;
@@ -1312,8 +1324,10 @@ k_boot2w(BootKArgs A) {
             }
         }
     } else {
-        LaneTw<double, 8> tw;
-        load_lane_tw(tw, (const C2<double>*)A.t1 + wave * 64 * 8, (const C2<double>*)A.t2, lane);
+        LaneTwS<double, 8> tw;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) tw.t1[r] = ((const C2<double>*)A.t1)[(wave * 64 + lane) * 8 + r];
+        tw.t2 = s_t2 + (lane & 7) * T2S;
         double* sx_own = s_xd[b][wave];
         const double* sx_other = s_xd[b][1 - wave];
         const C2<double>* key = (const C2<double>*)A.bk;
@@ -1496,7 +1510,7 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
             A.bk = key.bk64;
             A.t1 = tw.t1w64;
             A.t2 = tw.t2_64;
-            constexpr int GB = 2;   // 4 wavefronts: one workgroup per CU at one wavefront per SIMD
+            constexpr int GB = 4;   // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD
             hipLaunchKernelGGL((k_boot2w<3, 8, 0, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         } else {
             A.bk = key.bk32;

@@ -139,6 +139,14 @@ struct LaneTw {
     C2<T> t2[8];
 };
 
+// The same twiddles with the pass-2 set read from LDS at each use instead of held in registers:
+// t2 points at this lane's eight W_64^{c k2} (c = lane mod 8), shared by every lane with that c.
+template <typename T, int R1>
+struct LaneTwS {
+    C2<T> t1[R1];
+    const C2<T>* t2;
+};
+
 template <typename T, int R1>
 __device__ __forceinline__ void load_lane_tw(LaneTw<T, R1>& tw, const C2<T>* __restrict__ g_t1,
                                              const C2<T>* __restrict__ g_t2, int lane) {
@@ -212,8 +220,8 @@ __device__ __forceinline__ void exchange(C2<T>* v, void* buf, WR wr, RD rd) {
 // multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
 // lane order.  buf: xbuf_elems<M>() elements of C2<T> (of T when HALF) private to the wavefront.
 // UNIT0: tw.t1[0] is 1 (plain pass-1 twiddles, no twist folded in), so its multiply is skipped.
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false>
-__device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, void* vbuf, int lane) {
+template <int R1, typename T, bool HALF = false, bool UNIT0 = false, typename TW>
+__device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
     dftR<R1, +1>(v);
 #pragma unroll
@@ -269,8 +277,8 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const LaneTw<T, R1>& tw, void*
 
 // Inverse of fft_fwd without the 1/M scale and without the a-dependent untwist constants:
 // in: v in lane order; out: v[a] = M * z_{64a+lane} * (conj of the forward a-constant).
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false>
-__device__ __forceinline__ void fft_inv(C2<T>* v, const LaneTw<T, R1>& tw, void* vbuf, int lane) {
+template <int R1, typename T, bool HALF = false, bool UNIT0 = false, typename TW>
+__device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
 #pragma unroll
     for (int s = 0; s < G; ++s) dft8<-1>(v + s * 8);
