@@ -1172,9 +1172,9 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
     }
 }
 
-// FP64 step at N = 2048, one wavefront's share; exchanges through half-width buffers.
+// FP64 step at N = 2048, one wavefront's share.
 template <int L, int BG, bool ROT>
-__device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const double* sx_other, int a,
+__device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, const C2<double>* sx_other, int a,
                                          const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
                                          const LaneTwS<double, 8>& tw0,
                                          const C2<double>* __restrict__ rw_own, const C2<double>* __restrict__ rw_other,
@@ -1210,7 +1210,7 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
                 const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                 v[aa] = w2_prep(p0, p1, rw_own[aa], wave, aa == 0);
             }
-            fft_fwd<R1, double, true>(v, tw, sx_own, lane);
+            fft_fwd<R1, double>(v, tw, sx_own, lane);
             const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
 #pragma unroll
             for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64]);
@@ -1222,38 +1222,23 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
         }
     }
     asm volatile("" ::"v"(sink));
-    fft_inv<R1, double, true>(o0, tw, sx_own, lane);
-    fft_inv<R1, double, true>(o1, tw, sx_own, lane);
+    fft_inv<R1, double>(o0, tw, sx_own, lane);
+    fft_inv<R1, double>(o1, tw, sx_own, lane);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const C2<double>* X = q ? o1 : o0;
-        double yx[R1], yy[R1];
         if (work) {
 #pragma unroll
-            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].x;
-        }
-        __syncthreads();
-        if (work) {
-#pragma unroll
-            for (int aa = 0; aa < R1; ++aa) yx[aa] = sx_other[aa * 64 + lane];
-        }
-        __syncthreads();
-        if (work) {
-#pragma unroll
-            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa].y;
-        }
-        __syncthreads();
-        if (work) {
-#pragma unroll
-            for (int aa = 0; aa < R1; ++aa) yy[aa] = sx_other[aa * 64 + lane];
+            for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa];
         }
         __syncthreads();
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
             if (!work) break;
+            const C2<double> y = sx_other[aa * 64 + lane];
             const C2<double> own = aa ? cmulc(X[aa], rw_own[aa]) : X[aa];
-            const C2<double> oth = aa ? cmulc(C2<double>{yx[aa], yy[aa]}, rw_other[aa]) : C2<double>{yx[aa], yy[aa]};
+            const C2<double> oth = aa ? cmulc(y, rw_other[aa]) : y;
             const C2<double> z = w2_pair(own, oth, wave);
             const int j0 = q * N + w2_coef(wave, lane, aa, 0), j1 = q * N + w2_coef(wave, lane, aa, 1);
             if (raw) { raw[j0] = z.x; raw[j1] = z.y; }
@@ -1266,9 +1251,10 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, double* sx_own, const d
                 s_acc[j1] = (int32_t)c1;
             }
         }
+        // the partner's exchange buffer is read above and written again for the next polynomial,
+        // and both wavefronts read every accumulator coefficient at the start of the next step
+        __syncthreads();
     }
-    // both wavefronts read every accumulator coefficient at the start of the next step
-    __syncthreads();
 }
 
 // Bootstrap at N = 2048, two wavefronts per bootstrap, GB bootstraps per workgroup (their key reads
@@ -1281,7 +1267,7 @@ k_boot2w(BootKArgs A) {
     constexpr int XB = xbuf_elems<W2_H>();
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
     __shared__ int32_t s_acc_all[GB][2 * N];
-    __shared__ double s_xd[GB][2][XB];   // per wavefront: C2<float>[XB] (8 bytes each) or double[XB]
+    __shared__ double s_xd[GB][2][P ? XB : 2 * XB];   // per wavefront: C2<float>[XB] or C2<double>[XB]
     __shared__ int16_t s_bara_all[GB][kMaxLweN];
     __shared__ C2<double> s_t2[P ? 1 : 8 * T2S];
     const int lane = threadIdx.x & 63;
@@ -1328,8 +1314,8 @@ k_boot2w(BootKArgs A) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) tw.t1[r] = ((const C2<double>*)A.t1)[(wave * 64 + lane) * 8 + r];
         tw.t2 = s_t2 + (lane & 7) * T2S;
-        double* sx_own = s_xd[b][wave];
-        const double* sx_other = s_xd[b][1 - wave];
+        C2<double>* sx_own = (C2<double>*)s_xd[b][wave];
+        const C2<double>* sx_other = (const C2<double>*)s_xd[b][1 - wave];
         const C2<double>* key = (const C2<double>*)A.bk;
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
