@@ -85,19 +85,49 @@ __device__ __forceinline__ C2<T> mul_w8_3(C2<T> a) {
     else return {(a.y - a.x) * r, (-a.x - a.y) * r};
 }
 
-// In-place 8-point DFT, X_k = sum_n x_n e^{S 2 pi i nk/8}, natural order in and out.
-template <int S, typename T>
+// b + r q for a real r
+template <typename T>
+__device__ __forceinline__ C2<T> caxpy(T r, C2<T> q, C2<T> b) {
+    return {__builtin_fma(r, q.x, b.x), __builtin_fma(r, q.y, b.y)};
+}
+__device__ __forceinline__ C2<float> caxpy(float r, C2<float> q, C2<float> b) {
+    return {__builtin_fmaf(r, q.x, b.x), __builtin_fmaf(r, q.y, b.y)};
+}
+
+// In-place 8-point DFT, X_k = sum_n x_n e^{S 2 pi i nk/8}, natural order in and out.  The odd
+// half carries e^{S i pi/4} and e^{3 S i pi/4}; both share the factor 1/sqrt(2), which FUSE
+// applies once, fused into the last stage (52 real operations, no plain multiplies).  Without
+// FUSE the factor is applied on the way in, which keeps fewer values live (for kernels at their
+// register limit).
+template <int S, typename T, bool FUSE = true>
 __device__ __forceinline__ void dft8(C2<T>* v) {
+    if constexpr (!FUSE) {
+        C2<T> a0 = v[0] + v[4], a4 = v[0] - v[4];
+        C2<T> a1 = v[1] + v[5], a5 = mul_w8<S>(v[1] - v[5]);
+        C2<T> a2 = v[2] + v[6], a6 = mul_i<S>(v[2] - v[6]);
+        C2<T> a3 = v[3] + v[7], a7 = mul_w8_3<S>(v[3] - v[7]);
+        C2<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = mul_i<S>(a1 - a3);
+        C2<T> b4 = a4 + a6, b6 = a4 - a6, b5 = a5 + a7, b7 = mul_i<S>(a5 - a7);
+        v[0] = b0 + b1; v[4] = b0 - b1;
+        v[2] = b2 + b3; v[6] = b2 - b3;
+        v[1] = b4 + b5; v[5] = b4 - b5;
+        v[3] = b6 + b7; v[7] = b6 - b7;
+        return;
+    }
+    const T r = (T)0.70710678118654752440084436210485;
     C2<T> a0 = v[0] + v[4], a4 = v[0] - v[4];
-    C2<T> a1 = v[1] + v[5], a5 = mul_w8<S>(v[1] - v[5]);
+    C2<T> a1 = v[1] + v[5], d5 = v[1] - v[5];
     C2<T> a2 = v[2] + v[6], a6 = mul_i<S>(v[2] - v[6]);
-    C2<T> a3 = v[3] + v[7], a7 = mul_w8_3<S>(v[3] - v[7]);
+    C2<T> a3 = v[3] + v[7], d7 = v[3] - v[7];
+    // e^{S i pi/4} d5 = r p5, e^{3 S i pi/4} d7 = r p7
+    const C2<T> p5 = S > 0 ? C2<T>{d5.x - d5.y, d5.x + d5.y} : C2<T>{d5.x + d5.y, d5.y - d5.x};
+    const C2<T> p7 = S > 0 ? C2<T>{-d7.x - d7.y, d7.x - d7.y} : C2<T>{d7.y - d7.x, -d7.x - d7.y};
     C2<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = mul_i<S>(a1 - a3);
-    C2<T> b4 = a4 + a6, b6 = a4 - a6, b5 = a5 + a7, b7 = mul_i<S>(a5 - a7);
+    C2<T> b4 = a4 + a6, b6 = a4 - a6, q5 = p5 + p7, q7 = mul_i<S>(p5 - p7);
     v[0] = b0 + b1; v[4] = b0 - b1;
     v[2] = b2 + b3; v[6] = b2 - b3;
-    v[1] = b4 + b5; v[5] = b4 - b5;
-    v[3] = b6 + b7; v[7] = b6 - b7;
+    v[1] = caxpy(r, q5, b4); v[5] = caxpy(-r, q5, b4);
+    v[3] = caxpy(r, q7, b6); v[7] = caxpy(-r, q7, b6);
 }
 
 // In-place 16-point DFT: one radix-2 DIF step, then two 8-point DFTs.
@@ -123,9 +153,9 @@ __device__ __forceinline__ void dft16(C2<T>* v) {
     for (int m = 0; m < 8; ++m) { v[2 * m] = e[m]; v[2 * m + 1] = o[m]; }
 }
 
-template <int R, int S, typename T>
+template <int R, int S, typename T, bool FUSE = true>
 __device__ __forceinline__ void dftR(C2<T>* v) {
-    if constexpr (R == 8) dft8<S>(v);
+    if constexpr (R == 8) dft8<S, T, FUSE>(v);
     else dft16<S>(v);
 }
 
@@ -220,10 +250,10 @@ __device__ __forceinline__ void exchange(C2<T>* v, void* buf, WR wr, RD rd) {
 // multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
 // lane order.  buf: xbuf_elems<M>() elements of C2<T> (of T when HALF) private to the wavefront.
 // UNIT0: tw.t1[0] is 1 (plain pass-1 twiddles, no twist folded in), so its multiply is skipped.
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false, typename TW>
+template <int R1, typename T, bool HALF = false, bool UNIT0 = false, bool FUSE = true, typename TW>
 __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
-    dftR<R1, +1>(v);
+    dftR<R1, +1, T, FUSE>(v);
 #pragma unroll
     for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
     const int c = lane & 7, k1lo = lane >> 3;
@@ -242,7 +272,7 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int 
         wave_sync();
 #pragma unroll
         for (int s = 0; s < G; ++s) {
-            dft8<+1>(v + s * 8);
+            dft8<+1, T, FUSE>(v + s * 8);
 #pragma unroll
             for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
 #pragma unroll
@@ -263,7 +293,7 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int 
                            [&](int r) { return x1_addr(k1lo + 8 * (r >> 3), r & 7, c); });
 #pragma unroll
         for (int s = 0; s < G; ++s) {
-            dft8<+1>(v + s * 8);
+            dft8<+1, T, FUSE>(v + s * 8);
 #pragma unroll
             for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmul(v[s * 8 + k2], tw.t2[k2]);
         }
@@ -272,16 +302,16 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int 
                            [&](int r) { return x2_addr(r >> 3, k2r, k1lo3, r & 7); });
     }
 #pragma unroll
-    for (int s = 0; s < G; ++s) dft8<+1>(v + s * 8);
+    for (int s = 0; s < G; ++s) dft8<+1, T, FUSE>(v + s * 8);
 }
 
 // Inverse of fft_fwd without the 1/M scale and without the a-dependent untwist constants:
 // in: v in lane order; out: v[a] = M * z_{64a+lane} * (conj of the forward a-constant).
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false, typename TW>
+template <int R1, typename T, bool HALF = false, bool UNIT0 = false, bool FUSE = true, typename TW>
 __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
 #pragma unroll
-    for (int s = 0; s < G; ++s) dft8<-1>(v + s * 8);
+    for (int s = 0; s < G; ++s) dft8<-1, T, FUSE>(v + s * 8);
     const int c = lane & 7, k1lo = lane >> 3;
     if constexpr (!HALF) {
         C2<T>* buf = (C2<T>*)vbuf;
@@ -302,7 +332,7 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
         for (int s = 0; s < G; ++s) {
 #pragma unroll
             for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
-            dft8<-1>(v + s * 8);
+            dft8<-1, T, FUSE>(v + s * 8);
 #pragma unroll
             for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo + 8 * s, b, c)] = v[s * 8 + b];
         }
@@ -321,14 +351,14 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
         for (int s = 0; s < G; ++s) {
 #pragma unroll
             for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
-            dft8<-1>(v + s * 8);
+            dft8<-1, T, FUSE>(v + s * 8);
         }
         exchange<R1, true>(v, vbuf, [&](int r) { return x1_addr(k1lo + 8 * (r >> 3), r & 7, c); },
                            [&](int k1) { return x1_addr(k1, k1lo, c); });
     }
 #pragma unroll
     for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
-    dftR<R1, -1>(v);
+    dftR<R1, -1, T, FUSE>(v);
 }
 
 // ---- integer stages ----
