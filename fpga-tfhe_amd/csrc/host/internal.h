@@ -21,6 +21,8 @@ void set_error(const std::string& msg);
 void flatten_bk(const LweBootstrappingKey* bk, int32_t* out);
 void flatten_ksk(const LweKeySwitchKey* ks, int32_t* out);
 bool flat_params_of(const TFheGateBootstrappingParameterSet* p, tfhe_flat_params* out);
+// time-domain bk of an upstream-shaped FFT-domain key (values in LagrangeHalfCPolynomial.data)
+int bk_from_lagrange_struct(const LweBootstrappingKeyFFT* bkfft, const tfhe_flat_params* p, int32_t* bk);
 
 // device key bound to a cloud key set (created on first use)
 tfhe_gpu_key* device_key_for(const TFheGateBootstrappingCloudKeySet* bk);

@@ -487,14 +487,23 @@ extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk
 extern "C" int tfhe_gpu_key_upload(const TFheGateBootstrappingCloudKeySet* bk, int32_t prec_mask, tfhe_gpu_key** out) {
     Ctx& c = ctx();
     std::lock_guard<std::recursive_mutex> g(c.mu);
-    if (!bk || !bk->params || !bk->bk || !out) return arg_fail("null cloud key");
+    if (!bk || !bk->params || !out) return arg_fail("null cloud key");
+    const bool fft_only = !bk->bk;
+    if (fft_only && (!bk->bkFFT || !bk->bkFFT->bkFFT || !bk->bkFFT->ks))
+        return arg_fail("cloud key set without a bootstrapping key");
     tfhe_flat_params fp;
     flat_params_of(bk->params, &fp);
     const size_t bk_elems = (size_t)fp.n * (fp.k + 1) * fp.l * (fp.k + 1) * fp.N;
     const size_t ksk_elems = (size_t)fp.k * fp.N * fp.ks_t * (1 << fp.ks_basebit) * (fp.n + 1);
     std::vector<int32_t> hb(bk_elems), hk(ksk_elems);
-    flatten_bk(bk->bk, hb.data());
-    flatten_ksk(bk->bk->ks, hk.data());
+    if (fft_only) {
+        const int rc = bk_from_lagrange_struct(bk->bkFFT, &fp, hb.data());
+        if (rc) return rc;
+        flatten_ksk(bk->bkFFT->ks, hk.data());
+    } else {
+        flatten_bk(bk->bk, hb.data());
+        flatten_ksk(bk->bk->ks, hk.data());
+    }
     tfhe_gpu_key* k = nullptr;
     int rc = tfhe_gpu_key_create(&fp, hb.data(), hk.data(), prec_mask, &k);
     if (rc) return rc;

@@ -97,6 +97,9 @@ def lib():
         "tfhe_gpu_set_fp64_shared_rows": (ctypes.c_int, [i32]),
         "tfhe_gpu_key_create": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32, P(_vp)]),
         "tfhe_gpu_key_upload": (ctypes.c_int, [_vp, i32, P(_vp)]),
+        "tfhe_bk_from_lagrange": (ctypes.c_int, [P(FlatParams), P(ctypes.c_double), _i32p]),
+        "tfhe_bk_to_lagrange": (ctypes.c_int, [P(FlatParams), _i32p, P(ctypes.c_double)]),
+        "tfhe_gpu_key_create_lagrange": (ctypes.c_int, [P(FlatParams), P(ctypes.c_double), _i32p, i32, P(_vp)]),
         "tfhe_gpu_key_free": (ctypes.c_int, [_vp]),
         "tfhe_gpu_key_params": (ctypes.c_int, [_vp, P(FlatParams)]),
         "tfhe_gpu_gate_batch": (ctypes.c_int, [_vp, i32, i32, _i32p, _i32p, _i32p, _i32p, i32]),
@@ -232,6 +235,27 @@ def set_split_bits(b):
     _ck(lib().tfhe_gpu_set_split_bits(int(b)), "set_split_bits")
 
 
+def bk_to_lagrange(flat_params, bk):
+    """FFT-domain values (upstream LagrangeHalfC order, torus units) of every bk polynomial."""
+    fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+    bk = np.ascontiguousarray(bk, dtype=np.int32)
+    out = np.zeros(bk.shape[:-1] + (fp.N // 2,), np.complex128)
+    _ck(lib().tfhe_bk_to_lagrange(ctypes.byref(fp), i32p(bk), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+        "bk_to_lagrange")
+    return out
+
+
+def bk_from_lagrange(flat_params, lagrange):
+    """Exact time-domain bk from FFT-domain values; raises TfheError when they are not the
+    transform of torus32 polynomials."""
+    fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+    lag = np.ascontiguousarray(lagrange, dtype=np.complex128)
+    out = np.zeros(lag.shape[:-1] + (fp.N,), np.int32)
+    _ck(lib().tfhe_bk_from_lagrange(ctypes.byref(fp), lag.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), i32p(out)),
+        "bk_from_lagrange")
+    return out
+
+
 def set_fp64_shared_rows(on):
     _ck(lib().tfhe_gpu_set_fp64_shared_rows(1 if on else 0), "set_fp64_shared_rows")
 
@@ -249,6 +273,21 @@ class DeviceKey:
     @classmethod
     def from_keyset(cls, ks: KeySet, prec_mask=PREC_MASK_FP64 | PREC_MASK_SPLIT32):
         return cls(ks.fp, ks.bk, ks.ksk, prec_mask)
+
+    @classmethod
+    def from_lagrange(cls, flat_params, lagrange, ksk, prec_mask=PREC_MASK_FP64 | PREC_MASK_SPLIT32):
+        """Device key from an FFT-domain bk in upstream LagrangeHalfC order (complex128
+        [n][(k+1)l][k+1][N/2], torus units), as tfhe_gpu_key_create_lagrange."""
+        self = cls.__new__(cls)
+        fp = FlatParams(**flat_params) if isinstance(flat_params, dict) else flat_params
+        self.fp = fp
+        self.ptr = ctypes.c_void_p()
+        lag = np.ascontiguousarray(lagrange, dtype=np.complex128)
+        ksk = np.ascontiguousarray(ksk, dtype=np.int32)
+        _ck(lib().tfhe_gpu_key_create_lagrange(ctypes.byref(fp), lag.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               i32p(ksk), prec_mask, ctypes.byref(self.ptr)),
+            "tfhe_gpu_key_create_lagrange")
+        return self
 
     def free(self):
         if self.ptr:

@@ -318,7 +318,24 @@ int tfhe_gpu_sync(void);
  * without its h=0 rows. */
 int tfhe_gpu_key_create(const tfhe_flat_params* p, const int32_t* bk, const int32_t* ksk,
                         int32_t prec_mask, tfhe_gpu_key** out);
+/* Device key of a cloud key set.  A key set made by this library (keygen or file import) brings
+ * its time-domain bk; an upstream-shaped one with bk == NULL brings its FFT-domain key instead:
+ * bkFFT->bkFFT[i].all_samples[r].a[q].data = N/2 complex doubles in upstream LagrangeHalfC order
+ * (host_to_fpga.h:65-72), plus bkFFT->ks, converted exactly as by tfhe_bk_from_lagrange. */
 int tfhe_gpu_key_upload(const TFheGateBootstrappingCloudKeySet* bk, int32_t prec_mask, tfhe_gpu_key** out);
+/* FFT-domain bootstrapping keys in upstream order.  An FFT-domain key is the flat array
+ * [n][(k+1)l][k+1][N/2] complex double (re, im) of each key polynomial t's values
+ * L_i = sum_j (t_j / 2^32) e^{i pi (2i+1) j / N}, i < N/2: the LagrangeHalfCPolynomial order and
+ * torus units of upstream, fixed by the reference's conversions (tGswFFTExternMulToTLwe.cpp:97-130,
+ * TorusPolynomial_fft.cpp:4-43).  It replaces reading bkFFT coefsC at host_to_fpga.h:65-72.
+ * tfhe_bk_from_lagrange recovers the exact time-domain bk [n][(k+1)l][k+1][N] int32 (TFHE_E_ARG,
+ * with the polynomial named, when a value is not within 1/4 of an integer after the inverse
+ * transform); tfhe_bk_to_lagrange is its inverse; tfhe_gpu_key_create_lagrange is
+ * tfhe_gpu_key_create from an FFT-domain bk.  Host-side conversions, any power-of-two N. */
+int tfhe_bk_from_lagrange(const tfhe_flat_params* p, const double* lagrange, int32_t* bk);
+int tfhe_bk_to_lagrange(const tfhe_flat_params* p, const int32_t* bk, double* lagrange);
+int tfhe_gpu_key_create_lagrange(const tfhe_flat_params* p, const double* lagrange, const int32_t* ksk,
+                                 int32_t prec_mask, tfhe_gpu_key** out);
 int tfhe_gpu_key_free(tfhe_gpu_key* key);
 int tfhe_gpu_key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 /* split point of the SPLIT32 key: key = hi * 2^split + lo.  Applies to keys created afterwards;

@@ -104,6 +104,7 @@ def lib():
             "orc_fft_forward": (None, [f64p, i32p, i32]),
             "orc_fft_inverse_raw": (None, [f64p, f64p, i32]),
             "orc_lagrange_half_reference": (None, [f64p, i32p, i32]),
+            "orc_torus_from_lagrange_reference": (None, [i32p, f64p, i32]),
             "orc_extern_product_exact": (None, [pp, i32p, i32p, i64p]),
             "orc_fft_key_create": (vp, [pp, i32p]),
             "orc_fft_key_free": (None, [vp]),
@@ -306,6 +307,15 @@ def lagrange_half_reference(poly):
     out = np.zeros(len(poly), dtype=np.float64)
     lib().orc_lagrange_half_reference(f64p(out), i32p(poly), len(poly))
     return out[0::2] + 1j * out[1::2]
+
+
+def torus_from_lagrange_reference(values):
+    """TorusPolynomial_fft.cpp:4-43 on N/2 complex values (truncating)."""
+    v = np.ascontiguousarray(values, dtype=np.complex128)
+    N = 2 * len(v)
+    out = np.zeros(N, dtype=np.int32)
+    lib().orc_torus_from_lagrange_reference(i32p(out), f64p(v.view(np.float64)), N)
+    return out
 
 
 # ---- external product / bootstrapping ----

@@ -420,6 +420,29 @@ extern "C" void orc_lagrange_half_reference(double* out, const int32_t* poly, in
     }
 }
 
+/* TorusPolynomial_fft.cpp:4-43, the reference's way back from the FFT domain: the N/2 values p_i
+ * go to the odd slots 2i+1 of a 2N-point vector and their conjugates to slots 2N-1-2i, a 2N-point
+ * DFT with kernel e^{-2 pi i st/2N} (twidI.txt sign) is taken, and coefficient t is
+ * int32(int64(Re(out_t) / N * 2^32)) -- truncation toward zero, as there.  Restated by direct
+ * summation; `in` holds N/2 complex values (re, im interleaved). */
+extern "C" void orc_torus_from_lagrange_reference(int32_t* out, const double* in, int32_t N) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int t = 0; t < N; ++t) {
+        long double re = 0;
+        for (int i = 0; i < N / 2; ++i) {
+            const long double pr = in[2 * i], pi_ = in[2 * i + 1];
+            /* slot s = 2i+1 holds p_i, slot 2N-1-2i holds conj(p_i); angle -2 pi s t / 2N, mod 2N */
+            const long long s1 = ((long long)(2 * i + 1) * t) % (2LL * N);
+            const long long s2 = ((long long)(2 * N - 1 - 2 * i) * t) % (2LL * N);
+            const long double a1 = -pi * s1 / N, a2 = -pi * s2 / N;
+            re += pr * cosl(a1) - pi_ * sinl(a1);     /* Re(p e^{i a1}) */
+            re += pr * cosl(a2) + pi_ * sinl(a2);     /* Re(conj(p) e^{i a2}) */
+        }
+        const long double x = re / N * 4294967296.0L;
+        out[t] = (int32_t)(uint32_t)(int64_t)x;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* external product                                                         */
 /* ------------------------------------------------------------------------ */

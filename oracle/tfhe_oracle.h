@@ -69,6 +69,7 @@ void orc_negacyclic_mul_naive(int32_t* res, const int32_t* ipoly, const int32_t*
 void orc_fft_forward(double* out_re_im, const int32_t* poly, int32_t N);
 void orc_fft_inverse_raw(double* out, const double* in_re_im, int32_t N);
 void orc_lagrange_half_reference(double* out_re_im, const int32_t* poly, int32_t N);
+void orc_torus_from_lagrange_reference(int32_t* out, const double* in_re_im, int32_t N);
 
 /* ---- external product / blind rotation ---- */
 void orc_extern_product_exact(const orc_params* p, int32_t* acc, const int32_t* tgsw, int64_t* raw);
