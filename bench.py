@@ -242,12 +242,17 @@ def cpu_baseline(ks, batch_bits, seconds):
         ca, cb = ks.encrypt(batch_bits[0][:cnt]), ks.encrypt(batch_bits[1][:cnt])
         out, sec = O.gate_batch(keys, "NAND", ca, cb, nthreads=threads)
         ok = bool(np.array_equal(ks.decrypt(out), 1 - (batch_bits[0][:cnt] & batch_bits[1][:cnt])))
+        # the same path on one core, over a sample of about a fifth of the time
+        one = max(1, int(seconds / 5 * cnt / sec / threads))
+        out1, sec1 = O.gate_batch(keys, "NAND", ca[:one], cb[:one], nthreads=1)
+        ok = ok and bool(np.array_equal(out1, out[:one]))
         try:
             model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")][0]
         except Exception:
             model = "unknown"
         return {"value": cnt / sec, "unit": "gates/s", "cores": threads, "kind": "port",
                 "sample": f"{cnt} NAND gates (bootstrap + keyswitch, FP64 FFT path), {sec:.1f} s, {model}",
+                "single_thread": {"value": one / sec1, "unit": "gates/s", "sample": f"{one} of the same gates, {sec1:.1f} s"},
                 "correct": ok}
     finally:
         keys.free()
