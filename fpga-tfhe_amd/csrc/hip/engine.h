@@ -78,7 +78,7 @@ struct DevKeyView {
     const void* bk64 = nullptr;      // FP64 FFT-domain key, lane order
     const void* bk32 = nullptr;      // split FP32 key {hi.re, hi.im, lo.re, lo.im}
     int split_bits = 24;
-    const int32_t* ksk = nullptr;    // [kN][t][base-1][NP]
+    const int32_t* ksk = nullptr;    // [kN][t/2][15][NP] digit-pair rows
     int ksk_np = 0;                  // padded output length (multiple of 64)
 };
 
@@ -98,7 +98,9 @@ hipError_t launch_keyswitch(const DevKeyView& key, int32_t* out, const int32_t* 
 // time-domain bk [n][2l][2][N] (device) -> FP64 and/or split FP32 FFT-domain keys
 hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32_t* d_bk, void* bk64, void* bk32,
                           int split_bits, hipStream_t st);
-// KSK [kN][t][base][n+1] (device) -> [kN][t][base-1][NP]
+// KSK [kN][t][base][n+1] (device) -> [kN][t/2][15][NP]: row 4 d1 + d2 - 1 of pair g holds
+// ks[2g][d1] + ks[2g+1][d2] (base 4 only)
+size_t ksk_device_bytes(const DevParams& p, int np);
 hipError_t launch_ksk_relayout(const DevParams& p, const int32_t* d_ksk_in, int32_t* d_ksk_out, int np,
                                hipStream_t st);
 hipError_t launch_lwe_linear(const LinDesc* d_descs, int ndesc, int count, int n, hipStream_t st);

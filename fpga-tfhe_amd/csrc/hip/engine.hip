@@ -7,7 +7,7 @@
 //   k_keyswitch          bit-exact LWE keyswitch, a tile of G ciphertexts per workgroup so each
 //                        keyswitch-key row is read once per tile (coalesced over coefficients).
 //   k_keyprep            time-domain bootstrapping key -> FFT domain (FP64 or FP32 hi/lo).
-//   k_ksk_relayout       keyswitch key -> [kN][t][base-1][NP] without the unused h=0 rows.
+//   k_ksk_relayout       keyswitch key -> [kN][t/2][15][NP] sums of digit-pair rows.
 //   k_extract, k_rotdec, k_fft_test, k_decrypt  stage kernels for parity tests and audits.
 #include <cmath>
 #include <type_traits>
@@ -553,30 +553,37 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
 
 // ---------------------------------------------------------------------------------------------
 // Keyswitch (lwe-keyswitch-functions.cpp:114-129, :228-241), bit-exact.
+// The t base-4 digits of each input coefficient are taken in pairs: the key holds, for every
+// input index i and digit pair g, the 15 sums ks[i][2g][d1] + ks[i][2g+1][d2] over (d1, d2) != 0
+// (ks[.][.][0] = 0, the row the reference skips), so one row subtraction replaces two.  Integer
+// sums wrap mod 2^32 as the reference's two subtractions do, so the result is bit-identical.
 // Workgroup = KS_GATES ciphertexts (one per lane) x a chunk of C output coefficients (registers).
-// For each input index i the workgroup stages the t*3 candidate rows ks[i][j][1..3][chunk] in LDS
-// (row 0 is a zero row standing for digit 0, which the reference skips), and every lane subtracts
-// row[j][digit] for its own digit: one LDS read and one subtract per (ciphertext, i, j, coef), with
-// each keyswitch-key row read from memory once per workgroup.  The extracted samples are staged
-// through LDS in blocks of KS_IBLK indices so their reads stay coalesced.
+// For each input index i the workgroup stages the (t/2)*15 rows of the chunk in LDS, with a zero
+// row for the pair (0, 0), and every lane subtracts the row its own 4-bit digit pair selects: one
+// LDS read and one subtract per (ciphertext, i, pair, coefficient), each key row read from memory
+// once per workgroup.  The extracted samples are staged through LDS in blocks of KS_IBLK indices
+// so their reads stay coalesced.
 // ---------------------------------------------------------------------------------------------
 constexpr int KS_GATES = 256;
 constexpr int KS_IBLK = 16;
+constexpr int KS_PAIR_ROWS = 15;   // (d1, d2) in [0,4)^2 without (0, 0)
 
 template <int TKS, int KN, int NP, int C>
 __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
                                                         int32_t add_b, const int32_t* __restrict__ ksk,
                                                         int32_t* __restrict__ out, int count, int n,
                                                         int32_t* const* __restrict__ out_tab, int out_seg) {
-    static_assert(KS_GATES % C == 0 && NP % C == 0 && KN % KS_IBLK == 0, "keyswitch tiling");
-    // candidate rows, each padded by one uint4 so the 4 rows of a (j) start in distinct banks
+    static_assert(KS_GATES % C == 0 && NP % C == 0 && KN % KS_IBLK == 0 && TKS % 2 == 0, "keyswitch tiling");
+    constexpr int TP = TKS / 2;                      // digit pairs
+    constexpr int RPI = TP * KS_PAIR_ROWS;           // key rows per input index
+    // rows padded by one uint4 (36 dwords apart): the 16 rows of a pair start on distinct bank quads
     constexpr int KROW = C / 4 + 1;
-    __shared__ uint4 s_k[2][TKS][4][KROW];
+    __shared__ uint4 s_k[2][TP][16][KROW];
     __shared__ uint32_t s_u[KS_IBLK][KS_GATES];
     constexpr size_t UST = (size_t)KN + 1;
-    constexpr int KV = TKS * 3 * C;                  // staged values per input index
+    constexpr int KV = RPI * C;                      // staged values per input index
     constexpr int KPT = (KV + KS_GATES - 1) / KS_GATES;
-    constexpr int ROWS_PER_T = KS_GATES / C;         // (j, h) rows covered by one pass of the block
+    constexpr int ROWS_PER_T = KS_GATES / C;         // rows covered by one pass of the block
     const int tid = threadIdx.x;
     const int g0 = blockIdx.y * KS_GATES;
     const int g = g0 + tid;
@@ -586,8 +593,8 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
     uint32_t acc[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[c] = 0;
-    for (int idx = tid; idx < 2 * TKS * (C / 4); idx += KS_GATES) {
-        const int b = idx / (TKS * (C / 4)), r = idx % (TKS * (C / 4));
+    for (int idx = tid; idx < 2 * TP * (C / 4); idx += KS_GATES) {
+        const int b = idx / (TP * (C / 4)), r = idx % (TP * (C / 4));
         s_k[b][r / (C / 4)][0][r % (C / 4)] = make_uint4(0, 0, 0, 0);
     }
     // staged value v = tid + t * KS_GATES covers row jh = v / C = tid / C + t * ROWS_PER_T and
@@ -600,14 +607,15 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
 #pragma unroll
         for (int t = 0; t < KPT; ++t) {
             const int jh = jh0 + t * ROWS_PER_T;
-            pre[t] = (jh < TKS * 3) ? (uint32_t)kbase[((size_t)i * TKS * 3 + t * ROWS_PER_T) * NP] : 0u;
+            pre[t] = (jh < RPI) ? (uint32_t)kbase[((size_t)i * RPI + t * ROWS_PER_T) * NP] : 0u;
         }
     };
     auto store_rows = [&](int b) {
 #pragma unroll
         for (int t = 0; t < KPT; ++t) {
             const int jh = jh0 + t * ROWS_PER_T;
-            if (jh < TKS * 3) sk_flat[((b * TKS + jh / 3) * 4 + jh % 3 + 1) * (KROW * 4) + cc] = pre[t];
+            if (jh < RPI)
+                sk_flat[((b * TP + jh / KS_PAIR_ROWS) * 16 + jh % KS_PAIR_ROWS + 1) * (KROW * 4) + cc] = pre[t];
         }
     };
     load_rows(0);
@@ -636,12 +644,13 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
             const int b = i & 1;
             if (i + 1 < KN) load_rows(i + 1);
             const uint32_t aibar = s_u[ii][tid];
-            // not unrolled: an unrolled j loop lets the compiler re-associate the eight row
-            // subtractions into one sum and keep all eight rows live (four times the registers)
+            // not unrolled: an unrolled pair loop lets the compiler re-associate the row
+            // subtractions into one sum and keep every row live
 #pragma unroll 1
-            for (int j = 0; j < TKS; ++j) {
-                const uint32_t d = (aibar >> (30 - 2 * j)) & 3u;
-                const uint4* row = &s_k[b][j][d][0];
+            for (int jp = 0; jp < TP; ++jp) {
+                // digits 2jp (high two bits) and 2jp+1: (aibar >> (32 - 2(2jp+2))) & 15
+                const uint32_t d = (aibar >> (28 - 4 * jp)) & 15u;
+                const uint4* row = &s_k[b][jp][d][0];
 #pragma unroll
                 for (int c4 = 0; c4 < C / 4; ++c4) {
                     const uint4 kv = row[c4];
@@ -723,15 +732,25 @@ __global__ void __launch_bounds__(64) k_keyprep(const int32_t* __restrict__ bk, 
 
 __global__ void k_ksk_relayout(const int32_t* __restrict__ in, int32_t* __restrict__ out, int kN, int t, int n,
                                int np) {
-    // out[i][j][h-1][c] = in[i][j][h][c] for h in 1..3, c <= n; zero padding for c > n
-    const size_t total = (size_t)kN * t * 3 * np;
+    // out[i][g][4 d1 + d2 - 1][c] = in[i][2g][d1][c] + in[i][2g+1][d2][c] (mod 2^32) for
+    // (d1, d2) != (0, 0), with in[.][.][0] taken as zero; c <= n, zero padding for c > n
+    const size_t total = (size_t)kN * (t / 2) * KS_PAIR_ROWS * np;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int c = (int)(idx % np);
-        const size_t r = idx / np;  // (i*t + j)*3 + (h-1)
-        const int hm1 = (int)(r % 3);
-        const size_t ij = r / 3;
-        out[idx] = (c <= n) ? in[(ij * 4 + (hm1 + 1)) * (size_t)(n + 1) + c] : 0;
+        const size_t r = idx / np;  // (i*(t/2) + g)*15 + hh - 1
+        const int hh = (int)(r % KS_PAIR_ROWS) + 1, d1 = hh >> 2, d2 = hh & 3;
+        const size_t ig = r / KS_PAIR_ROWS;         // i*(t/2) + g, so digit 2g of index i is row 2*ig
+        uint32_t v = 0;
+        if (c <= n) {
+            if (d1) v += (uint32_t)in[((2 * ig) * 4 + d1) * (size_t)(n + 1) + c];
+            if (d2) v += (uint32_t)in[((2 * ig + 1) * 4 + d2) * (size_t)(n + 1) + c];
+        }
+        out[idx] = (int32_t)v;
     }
+}
+
+size_t ksk_device_bytes(const DevParams& p, int np) {
+    return (size_t)p.N * (p.ks_t / 2) * KS_PAIR_ROWS * np * sizeof(int32_t);
 }
 
 // LWE linear jobs of a circuit level (NOT, COPY, constants): one workgroup per (desc, sample).

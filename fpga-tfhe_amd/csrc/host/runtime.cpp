@@ -452,7 +452,7 @@ extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk
     const int base = 1 << p.ks_basebit;
     const size_t ksk_in_bytes = (size_t)p.N * p.ks_t * base * (p.n + 1) * sizeof(int32_t);
     k->ksk_np = ((p.n + 1) + 63) / 64 * 64;
-    const size_t ksk_bytes = (size_t)p.N * p.ks_t * (base - 1) * k->ksk_np * sizeof(int32_t);
+    const size_t ksk_bytes = ksk_device_bytes(p, k->ksk_np);
     int32_t* d_bk = nullptr;
     int32_t* d_ksk_in = nullptr;
     auto cleanup_fail = [&](int code) {
