@@ -19,6 +19,7 @@ struct DevTwiddles {
     void* t2_64 = nullptr;       // [64][8]  C2<double>: W_64^{(L&7) k2}
     void* t1_plain32 = nullptr;  // [64][R1] C2<float>: W_M^{L k1}     (split path)
     void* t2_32 = nullptr;       // [64][8]  C2<float>
+    void* t2dit32 = nullptr;     // [64][8]  C2<float>: W_64^{(L>>3) b}  (split path, pass-2 inputs)
     void* twist32 = nullptr;     // [64][R1] C2<float>:  w^{64a + L}      (split path, forward twist)
     void* untw64 = nullptr;      // [64][R1] C2<double>: w^{64a + L} / M  (split path, untwist)
     void* wa64 = nullptr;        // [R1]     C2<double>: w^{64a}

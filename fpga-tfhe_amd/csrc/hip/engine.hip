@@ -435,9 +435,10 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
 // ---------------------------------------------------------------------------------------------
 template <int N, int L, int BG, bool ROT>
 __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, const float4* __restrict__ kr,
-                                       const LaneTw<float, N / 128>& tw, const C2<float>* tws,
+                                       const DitTw<float>& tw, const C2<float>* tws,
                                        const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
+    static_assert(M == 512, "the split path runs the 512-point decimation-in-time transform");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
 #pragma unroll
@@ -458,9 +459,8 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             C2<float> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
-                v[aa] = cmul(C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)},
-                             tws[aa]);
-            fft_fwd<R1, float, false, true>(v, tw, s_buf, lane);
+                v[aa] = C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)};
+            fft512_dit<+1, true>(v, tw, tws, s_buf, lane);
             const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
@@ -476,10 +476,10 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             }
         }
     }
-    fft_inv<R1, float, false, true>(h0, tw, s_buf, lane);
-    fft_inv<R1, float, false, true>(l0, tw, s_buf, lane);
-    fft_inv<R1, float, false, true>(h1, tw, s_buf, lane);
-    fft_inv<R1, float, false, true>(l1, tw, s_buf, lane);
+    fft512_dit<-1, false>(h0, tw, (const C2<float>*)nullptr, s_buf, lane);
+    fft512_dit<-1, false>(l0, tw, (const C2<float>*)nullptr, s_buf, lane);
+    fft512_dit<-1, false>(h1, tw, (const C2<float>*)nullptr, s_buf, lane);
+    fft512_dit<-1, false>(l1, tw, (const C2<float>*)nullptr, s_buf, lane);
     const double hs = ldexp(1.0, split_bits);
     // the untwist factors are re-read every step (an opaque offset keeps the compiler from
     // hoisting them out of the blind-rotation loop into 32 long-lived registers)
@@ -526,8 +526,8 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
     boot_prologue<N>(A, g, lane, s_acc, s_bara);
     wave_sync();
 
-    LaneTw<float, R1> tw;
-    load_lane_tw(tw, (const C2<float>*)A.t1, (const C2<float>*)A.t2, lane);
+    DitTw<float> tw;
+    load_dit_tw(tw, (const C2<float>*)A.t2, (const C2<float>*)A.t1, lane);
     C2<float> tws[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
@@ -861,15 +861,17 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
     __shared__ C2<float> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
     const size_t g = blockIdx.x;
-    LaneTw<float, R1> tw;
-    load_lane_tw(tw, t1, t2, lane);
-    C2<float> v[R1];
+    DitTw<float> tw;
+    load_dit_tw(tw, t2, t1, lane);
+    C2<float> v[R1], tws[R1];
     if (!inverse) {
         const int32_t* p = (const int32_t*)in + g * N;
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa)
-            v[aa] = cmul(C2<float>{(float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]}, twist32[lane * R1 + aa]);
-        fft_fwd<R1, float, false, true>(v, tw, s_buf, lane);
+        for (int aa = 0; aa < R1; ++aa) {
+            v[aa] = C2<float>{(float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]};
+            tws[aa] = twist32[lane * R1 + aa];
+        }
+        fft512_dit<+1, true>(v, tw, (const C2<float>*)tws, s_buf, lane);
         double* o = out + g * N;
 #pragma unroll
         for (int s = 0; s < R1 / 8; ++s)
@@ -888,7 +890,7 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
                 const int k = lane_order_k<R1>(lane, s, k3);
                 v[s * 8 + k3] = C2<float>{(float)F[2 * k], (float)F[2 * k + 1]};
             }
-        fft_inv<R1, float, false, true>(v, tw, s_buf, lane);
+        fft512_dit<-1, false>(v, tw, (const C2<float>*)nullptr, s_buf, lane);
         double* o = out + g * N;
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
@@ -943,7 +945,7 @@ static void cis_pi_over_N(long long num, int N, double* re, double* im) {
 hipError_t build_twiddles(DevTwiddles& tw, int N) {
     const int M = N / 2, R1 = M / 64;
     std::vector<double> t1f(2 * 64 * R1), t2(2 * 64 * 8), untw(2 * 64 * R1), wa(2 * R1);
-    std::vector<float> t1p(2 * 64 * R1), t2f(2 * 64 * 8), tws(2 * 64 * R1);
+    std::vector<float> t1p(2 * 64 * R1), t2f(2 * 64 * 8), t2d(2 * 64 * 8), tws(2 * 64 * R1);
     for (int L = 0; L < 64; ++L) {
         for (int k1 = 0; k1 < R1; ++k1) {
             double re, im;
@@ -961,6 +963,9 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
             cis_pi_over_N((long long)(2 * N / 64) * (L & 7) * k2, N, &re, &im);
             t2[2 * (L * 8 + k2)] = re; t2[2 * (L * 8 + k2) + 1] = im;
             t2f[2 * (L * 8 + k2)] = (float)re; t2f[2 * (L * 8 + k2) + 1] = (float)im;
+            // decimation-in-time pass-2 input twiddle W_64^{(L>>3) b}, b = k2
+            cis_pi_over_N((long long)(2 * N / 64) * (L >> 3) * k2, N, &re, &im);
+            t2d[2 * (L * 8 + k2)] = (float)re; t2d[2 * (L * 8 + k2) + 1] = (float)im;
         }
     }
     for (int a = 0; a < R1; ++a) {
@@ -1001,6 +1006,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     if ((e = up(&tw.t2_64, t2.data(), t2.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.t1_plain32, t1p.data(), t1p.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.t2_32, t2f.data(), t2f.size() * 4)) != hipSuccess) return e;
+    if ((e = up(&tw.t2dit32, t2d.data(), t2d.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.twist32, tws.data(), tws.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.untw64, untw.data(), untw.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
@@ -1014,7 +1020,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
 }
 
 void free_twiddles(DevTwiddles& tw) {
-    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.twist32, tw.untw64, tw.wa64,
+    void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.t2dit32, tw.twist32, tw.untw64, tw.wa64,
                   tw.t1w64, tw.t1w32, tw.rw64, tw.rw32};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1491,8 +1497,8 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         hipLaunchKernelGGL((k_boot64<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
     } else {
         A.bk = key.bk32;
-        A.t1 = tw.t1_plain32;
-        A.t2 = tw.t2_32;
+        A.t1 = tw.t1_plain32;   // W_512^{c L}: pass-3 input twiddles of the decimation-in-time transform
+        A.t2 = tw.t2dit32;
         hipLaunchKernelGGL((k_boot32<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
     }
     return hipGetLastError();
@@ -1604,8 +1610,8 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
     } else {
         if (N == 1024)
             hipLaunchKernelGGL((k_fft_test32<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
-                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2_32, (const C2<float>*)tw.twist32,
-                               (const C2<double>*)tw.untw64);
+                               (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2dit32,
+                               (const C2<float>*)tw.twist32, (const C2<double>*)tw.untw64);
         else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test2w<1>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w32, tw.t2_32,
                                (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64);

@@ -94,6 +94,23 @@ __device__ __forceinline__ C2<float> caxpy(float r, C2<float> q, C2<float> b) {
     return {__builtin_fmaf(r, q.x, b.x), __builtin_fmaf(r, q.y, b.y)};
 }
 
+// Stages 2 and 3 of the 8-point DFT below (FUSE form), from the first-stage sums and differences
+// a_n = v_n + v_{n+4} (n < 4), a4 = v0 - v4, d5 = v1 - v5, a6 = e^{S i pi/2} (v2 - v6), d7 = v3 - v7.
+template <int S, typename T>
+__device__ __forceinline__ void dft8_tail(C2<T>* v, C2<T> a0, C2<T> a4, C2<T> a1, C2<T> d5, C2<T> a2, C2<T> a6,
+                                          C2<T> a3, C2<T> d7) {
+    const T r = (T)0.70710678118654752440084436210485;
+    // e^{S i pi/4} d5 = r p5, e^{3 S i pi/4} d7 = r p7
+    const C2<T> p5 = S > 0 ? C2<T>{d5.x - d5.y, d5.x + d5.y} : C2<T>{d5.x + d5.y, d5.y - d5.x};
+    const C2<T> p7 = S > 0 ? C2<T>{-d7.x - d7.y, d7.x - d7.y} : C2<T>{d7.y - d7.x, -d7.x - d7.y};
+    C2<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = mul_i<S>(a1 - a3);
+    C2<T> b4 = a4 + a6, b6 = a4 - a6, q5 = p5 + p7, q7 = mul_i<S>(p5 - p7);
+    v[0] = b0 + b1; v[4] = b0 - b1;
+    v[2] = b2 + b3; v[6] = b2 - b3;
+    v[1] = caxpy(r, q5, b4); v[5] = caxpy(-r, q5, b4);
+    v[3] = caxpy(r, q7, b6); v[7] = caxpy(-r, q7, b6);
+}
+
 // In-place 8-point DFT, X_k = sum_n x_n e^{S 2 pi i nk/8}, natural order in and out.  The odd
 // half carries e^{S i pi/4} and e^{3 S i pi/4}; both share the factor 1/sqrt(2), which FUSE
 // applies once, fused into the last stage (52 real operations, no plain multiplies).  Without
@@ -114,20 +131,47 @@ __device__ __forceinline__ void dft8(C2<T>* v) {
         v[3] = b6 + b7; v[7] = b6 - b7;
         return;
     }
-    const T r = (T)0.70710678118654752440084436210485;
-    C2<T> a0 = v[0] + v[4], a4 = v[0] - v[4];
-    C2<T> a1 = v[1] + v[5], d5 = v[1] - v[5];
-    C2<T> a2 = v[2] + v[6], a6 = mul_i<S>(v[2] - v[6]);
-    C2<T> a3 = v[3] + v[7], d7 = v[3] - v[7];
-    // e^{S i pi/4} d5 = r p5, e^{3 S i pi/4} d7 = r p7
-    const C2<T> p5 = S > 0 ? C2<T>{d5.x - d5.y, d5.x + d5.y} : C2<T>{d5.x + d5.y, d5.y - d5.x};
-    const C2<T> p7 = S > 0 ? C2<T>{-d7.x - d7.y, d7.x - d7.y} : C2<T>{d7.y - d7.x, -d7.x - d7.y};
-    C2<T> b0 = a0 + a2, b2 = a0 - a2, b1 = a1 + a3, b3 = mul_i<S>(a1 - a3);
-    C2<T> b4 = a4 + a6, b6 = a4 - a6, q5 = p5 + p7, q7 = mul_i<S>(p5 - p7);
-    v[0] = b0 + b1; v[4] = b0 - b1;
-    v[2] = b2 + b3; v[6] = b2 - b3;
-    v[1] = caxpy(r, q5, b4); v[5] = caxpy(-r, q5, b4);
-    v[3] = caxpy(r, q7, b6); v[7] = caxpy(-r, q7, b6);
+    dft8_tail<S>(v, v[0] + v[4], v[0] - v[4], v[1] + v[5], v[1] - v[5], v[2] + v[6], mul_i<S>(v[2] - v[6]),
+                 v[3] + v[7], v[3] - v[7]);
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// a * w, or a * conj(w) when CONJ
+template <bool CONJ, typename T>
+__device__ __forceinline__ C2<T> cmul_w(C2<T> a, C2<T> w) {
+    if constexpr (CONJ) return cmulc(a, w);
+    else return cmul(a, w);
+}
+
+// Radix-2 butterfly whose second input carries a twiddle: s = x0 + x1 w, d = x0 - x1 w (conj(w)
+// when CONJ).  Six FMAs, the difference taken as d = 2 x0 - s, instead of a four-instruction
+// complex multiply and four adds.
+template <bool CONJ, typename T>
+__device__ __forceinline__ void bfly_tw(C2<T> x0, C2<T> x1, C2<T> w, C2<T>& s, C2<T>& d) {
+    if constexpr (!CONJ) {
+        s.x = fmaT(x1.x, w.x, fmaT(-x1.y, w.y, x0.x));
+        s.y = fmaT(x1.x, w.y, fmaT(x1.y, w.x, x0.y));
+    } else {
+        s.x = fmaT(x1.x, w.x, fmaT(x1.y, w.y, x0.x));
+        s.y = fmaT(x1.y, w.x, fmaT(-x1.x, w.y, x0.y));
+    }
+    d.x = fmaT((T)2, x0.x, -s.x);
+    d.y = fmaT((T)2, x0.y, -s.y);
+}
+
+// The 8-point DFT (FUSE form) of the twiddled inputs v_n w_n (conj(w_n) when CONJ): each twiddle is
+// fused into a first-stage butterfly (bfly_tw).  UNIT0: w_0 = 1 and w[0] is not read.  With
+// UNIT0, 72 real operations against 80 for seven complex multiplies and dft8.
+template <int S, bool CONJ, bool UNIT0, typename T>
+__device__ __forceinline__ void dft8_tw(C2<T>* v, const C2<T>* w) {
+    C2<T> a0, a4, a1, d5, a2, d6, a3, d7;
+    bfly_tw<CONJ>(UNIT0 ? v[0] : cmul_w<CONJ>(v[0], w[0]), v[4], w[4], a0, a4);
+    bfly_tw<CONJ>(cmul_w<CONJ>(v[1], w[1]), v[5], w[5], a1, d5);
+    bfly_tw<CONJ>(cmul_w<CONJ>(v[2], w[2]), v[6], w[6], a2, d6);
+    bfly_tw<CONJ>(cmul_w<CONJ>(v[3], w[3]), v[7], w[7], a3, d7);
+    dft8_tail<S>(v, a0, a4, a1, d5, a2, mul_i<S>(d6), a3, d7);
 }
 
 // In-place 16-point DFT: one radix-2 DIF step, then two 8-point DFTs.
@@ -359,6 +403,59 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
 #pragma unroll
     for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
     dftR<R1, -1, T, FUSE>(v);
+}
+
+// Twiddles of the decimation-in-time 512-point transform (fft512_dit), per lane L; entry 0 of
+// each is 1 and never read.
+//   t2[b] = W_64^{b (L >> 3)}        pass-2 inputs, lane L = c + 8 k1
+//   t3[c] = W_512^{c L}              pass-3 inputs, lane L = k1 + 8 k2
+template <typename T>
+struct DitTw {
+    C2<T> t2[8];
+    C2<T> t3[8];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_dit_tw(DitTw<T>& tw, const C2<T>* __restrict__ g_t2,
+                                            const C2<T>* __restrict__ g_t3, int lane) {
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        tw.t2[r] = g_t2[lane * 8 + r];
+        tw.t3[r] = g_t3[lane * 8 + r];
+    }
+}
+
+// 512-point transform with natural order in and out (lane L, register r <-> index L + 64 r), for
+// both directions: S = +1 computes F_k = sum_j z_j W_512^{jk}; S = -1 runs the same factorisation
+// with every root conjugated, which gives M times the inverse.  Decimation in time over
+// j = 64a + 8b + c, k = k1 + 8 k2 + 64 k3, with
+//   F_k = sum_c W_8^{c k3} W_512^{c (k1 + 8 k2)} sum_b W_8^{b k2} W_64^{b k1} sum_a W_8^{a k1} z_j :
+//   pass 1  lane 8b+c,   registers a -> k1   (TWIST: the inputs are first multiplied by tws[a])
+//   pass 2  lane c+8k1,  registers b -> k2,  inputs times W_64^{b k1}
+//   pass 3  lane k1+8k2, registers c -> k3,  inputs times W_512^{c (k1+8k2)}
+// The exchanges are those of fft_fwd; every twiddle (and the twist) multiplies a butterfly input
+// and is fused into it (dft8_tw), so a transform costs 196 real operations per lane against 212
+// for fft_fwd with output twiddles.  buf: xbuf_elems<512>() elements private to the wavefront.
+template <int S, bool TWIST, typename T>
+__device__ __forceinline__ void fft512_dit(C2<T>* v, const DitTw<T>& tw, const C2<T>* tws, C2<T>* buf, int lane) {
+    constexpr bool CJ = S < 0;
+    if constexpr (TWIST) dft8_tw<S, false, false>(v, tws);
+    else dft8<S, T>(v);
+    const int lo = lane & 7, hi = lane >> 3;
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) buf[x1_addr(k1, hi, lo)] = v[k1];
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = buf[x1_addr(hi, b, lo)];
+    wave_sync();
+    dft8_tw<S, CJ, true>(v, tw.t2);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(0, k2, hi, lo)] = v[k2];
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[x2_addr(0, hi, lo, c)];
+    wave_sync();
+    dft8_tw<S, CJ, true>(v, tw.t3);
 }
 
 // ---- integer stages ----

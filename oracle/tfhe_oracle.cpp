@@ -584,7 +584,8 @@ extern "C" void orc_test_vector(const orc_params* p, int32_t* acc, int32_t barb,
     else for (int j = 0; j < N; ++j) b[j] = tv[j];
 }
 
-/* upstream tLweExtractLweSample at index 0 (absent from the reference):
+/* upstream tLweExtractLweSample at index 0 (absent from the reference, which never extracts;
+ * pinned by the closed form below and by the LWE phase = TLWE phase coefficient 0 identity):
  * u.a[q*N + 0] = acc.a[q][0], u.a[q*N + j] = -acc.a[q][N-j] (j >= 1), u.b = acc.b[0]. */
 extern "C" void orc_extract(const orc_params* p, int32_t* u, const int32_t* acc) {
     const int N = p->N, k = p->k;
