@@ -173,10 +173,11 @@ __device__ __forceinline__ void step64(int32_t* s_acc, void* s_buf, C2<double>* 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
+        const uint32_t base4 = rot_base4(lane, a);
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
             const int j = lane + 64 * r;
-            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
+            u[r] = (ROT ? rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, j, OFF) : (uint32_t)s_acc[q * N + j] + OFF) ^ OFF;
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
@@ -433,9 +434,20 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
 // factors, which are re-read each step instead of held in registers; the hi product is small
 // enough to round to the exact integer, so the 2^s scaling does not amplify FP32 error.
 // ---------------------------------------------------------------------------------------------
+// Split-FP32 key reads through a buffer resource: the per-lane part of the address is one 32-bit
+// offset and the row offset is uniform (a scalar register), so no per-lane 64-bit address
+// arithmetic runs in the blind-rotation loop; reads past the end of the key return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t key_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float4 key_load(__amdgpu_buffer_rsrc_t rs, uint32_t lane_off, uint32_t row_off) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)lane_off, (int)row_off, 0));
+}
+
+// kr: the key buffer; row0: byte offset of this step's row set in it
 template <int N, int L, int BG, bool ROT>
-__device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, const float4* __restrict__ kr,
-                                       const DitTw<float>& tw, const C2<float>* tws,
+__device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, __amdgpu_buffer_rsrc_t kr,
+                                       uint32_t row0, const DitTw<float>& tw, const C2<float>* tws,
                                        const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
     static_assert(M == 512, "the split path runs the 512-point decimation-in-time transform");
@@ -449,10 +461,11 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
+        const uint32_t base4 = rot_base4(lane, a);
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
             const int j = lane + 64 * r;
-            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
+            u[r] = (ROT ? rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, j, OFF) : (uint32_t)s_acc[q * N + j] + OFF) ^ OFF;
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
@@ -461,16 +474,16 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             for (int aa = 0; aa < R1; ++aa)
                 v[aa] = C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)};
             fft512_dit<+1, true>(v, tw, tws, s_buf, lane);
-            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M;
+            const uint32_t ro = row0 + (uint32_t)((q * L + lvl) * 2 * M * sizeof(float4));
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
-                const float4 k0 = k[r * 64 + lane];
+                const float4 k0 = key_load(kr, (uint32_t)lane * sizeof(float4), ro + r * 64 * sizeof(float4));
                 cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
                 cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
             }
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
-                const float4 k1 = k[M + r * 64 + lane];
+                const float4 k1 = key_load(kr, (uint32_t)lane * sizeof(float4), ro + (M + r * 64) * sizeof(float4));
                 cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
                 cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
             }
@@ -532,20 +545,22 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
 #pragma unroll
     for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
     const C2<double>* untw = (const C2<double>*)A.untw64;
-    const float4* key = (const float4*)A.bk;
-    constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+    // one row set (2L rows of 2 polynomials of M float4) per LWE coefficient; the whole key is far
+    // below 4 GiB, so byte offsets fit 32 bits
+    constexpr uint32_t ROWSET_B = (uint32_t)(KPL * 2 * M * sizeof(float4));
+    const __amdgpu_buffer_rsrc_t key = key_rsrc(A.bk, (uint32_t)A.n * ROWSET_B);
     const int sb = A.split_bits;
 
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-        step32<N, L, BG, false>(s_acc, s_buf, 0, key + (size_t)A.L.row_set[g] * ROWSET, tw, tws, untw, lane, sb,
+        step32<N, L, BG, false>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, untw, lane, sb,
                                 raw);
     } else {
         const int n = A.n;
         for (int i = 0; i < n; ++i) {
             const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
             if (a == 0) continue;
-            step32<N, L, BG, true>(s_acc, s_buf, a, key + (size_t)i * ROWSET, tw, tws, untw, lane, sb, nullptr);
+            step32<N, L, BG, true>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb, nullptr);
         }
     }
     boot_epilogue<N>(A, g, lane, s_acc);
@@ -789,14 +804,21 @@ __global__ void __launch_bounds__(64) k_rotdec(int32_t* __restrict__ rot, int32_
     const int g = blockIdx.x, lane = threadIdx.x;
     for (int j = lane; j < 2 * N; j += 64) s_acc[j] = acc[(size_t)g * 2 * N + j];
     __syncthreads();
+    // the blind-rotation loop's own forms: rotdiff_e4 and the signed bit-field digits, with the
+    // plain forms (rotdiff, digit) as a cross-check that poisons the output on any disagreement
     const int a = bara[g];
     for (int q = 0; q < 2; ++q)
         for (int j = lane; j < N; j += 64) {
-            const uint32_t t = rotdiff<N>(s_acc + q * N, j, a);
-            rot[((size_t)g * 2 + q) * N + j] = (int32_t)t;
-            const uint32_t uu = t + OFF;
+            const uint32_t e4 = rot_base4(lane, a) + 256u * (uint32_t)(j >> 6);
+            const uint32_t t = rotdiff_e4<N>(s_acc + q * N, e4, j, 0);
+            const uint32_t ux = rotdiff_e4<N>(s_acc + q * N, e4, j, OFF) ^ OFF;
+            const bool same = t == rotdiff<N>(s_acc + q * N, j, a);
+            rot[((size_t)g * 2 + q) * N + j] = same ? (int32_t)t : (int32_t)(t ^ 0x5a5a5a5au);
 #pragma unroll
-            for (int lvl = 0; lvl < L; ++lvl) digits[(((size_t)g * 2 + q) * L + lvl) * N + j] = digit<BG>(uu, lvl);
+            for (int lvl = 0; lvl < L; ++lvl) {
+                const int d = digit_sx<BG>(ux, lvl);
+                digits[(((size_t)g * 2 + q) * L + lvl) * N + j] = d == digit<BG>(t + OFF, lvl) ? d : 1 << 20;
+            }
         }
 }
 

@@ -354,6 +354,16 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int 
 template <int R1, typename T, bool HALF = false, bool UNIT0 = false, bool FUSE = true, typename TW>
 __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
+    // conj(t2) on the inputs of an 8-point inverse DFT; FUSE folds each twiddle into a butterfly
+    auto inv_pass2 = [](C2<T>* x, const TW& t) {
+        if constexpr (FUSE) {
+            dft8_tw<-1, true, true>(x, t.t2);
+        } else {
+#pragma unroll
+            for (int k2 = 1; k2 < 8; ++k2) x[k2] = cmulc(x[k2], t.t2[k2]);
+            dft8<-1, T, false>(x);
+        }
+    };
 #pragma unroll
     for (int s = 0; s < G; ++s) dft8<-1, T, FUSE>(v + s * 8);
     const int c = lane & 7, k1lo = lane >> 3;
@@ -374,9 +384,7 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
         wave_sync();
 #pragma unroll
         for (int s = 0; s < G; ++s) {
-#pragma unroll
-            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
-            dft8<-1, T, FUSE>(v + s * 8);
+            inv_pass2(v + s * 8, tw);
 #pragma unroll
             for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo + 8 * s, b, c)] = v[s * 8 + b];
         }
@@ -392,17 +400,17 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
         exchange<R1, true>(v, vbuf, [&](int r) { return x2_addr(r >> 3, k2r, k1lo3, r & 7); },
                            [&](int r) { return x2_addr(r >> 3, r & 7, k1lo, c); });
 #pragma unroll
-        for (int s = 0; s < G; ++s) {
-#pragma unroll
-            for (int k2 = 1; k2 < 8; ++k2) v[s * 8 + k2] = cmulc(v[s * 8 + k2], tw.t2[k2]);
-            dft8<-1, T, FUSE>(v + s * 8);
-        }
+        for (int s = 0; s < G; ++s) inv_pass2(v + s * 8, tw);
         exchange<R1, true>(v, vbuf, [&](int r) { return x1_addr(k1lo + 8 * (r >> 3), r & 7, c); },
                            [&](int k1) { return x1_addr(k1, k1lo, c); });
     }
+    if constexpr (FUSE && R1 == 8) {
+        dft8_tw<-1, true, UNIT0>(v, tw.t1);
+    } else {
 #pragma unroll
-    for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
-    dftR<R1, -1, T, FUSE>(v);
+        for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
+        dftR<R1, -1, T, FUSE>(v);
+    }
 }
 
 // Twiddles of the decimation-in-time 512-point transform (fft512_dit), per lane L; entry 0 of
@@ -468,6 +476,21 @@ __device__ __forceinline__ uint32_t rotdiff(const int32_t* p, int j, int a) {
     uint32_t s = (uint32_t)p[e & (N - 1)];
     s = (e < N) ? s : (0u - s);
     return s - (uint32_t)p[j];
+}
+
+// The same coefficient plus `off` (the gadget offset of the decomposition that follows), written
+// for the instruction count of the blind-rotation loop.  Coefficient j = lane + 64 r takes the byte
+// offset e4 = base4 + 256 r, base4 = 4 (lane - a) (rot_base4, once per step): the address is
+// e4 mod 4N, and m = -1 when (j - a) mod 2N >= N (bit log2(4N) of e4, one signed bit-field
+// extract), so that +-p_e = (p_e ^ m) - m.
+__device__ __forceinline__ uint32_t rot_base4(int lane, int a) { return (uint32_t)(lane - a) << 2; }
+
+template <int N>
+__device__ __forceinline__ uint32_t rotdiff_e4(const int32_t* p, uint32_t e4, int j, uint32_t off) {
+    constexpr int SH = __builtin_ctz(4 * N);
+    const uint32_t s = *(const uint32_t*)((const char*)p + (e4 & (4u * N - 4u)));
+    const uint32_t m = (uint32_t)((int32_t)(e4 << (31 - SH)) >> 31);
+    return (s ^ m) + ((off - m) - (uint32_t)p[j]);
 }
 
 // signed gadget digit of level lvl (tGswTorus32PolynomialDecompH.cpp:29-44) of u = x + offset

@@ -67,6 +67,23 @@ def test_rotate_decompose_bit_exact(rng):
             assert np.array_equal(dig[g, q * P.l:(q + 1) * P.l], wd), (g, q)
 
 
+def test_rotate_decompose_every_rotation(rng):
+    """Every rotation amount a in [0, 2N) on one accumulator: the blind-rotation loop's byte-offset
+    rotation and signed-extract digits against the oracle (the kernel also cross-checks them
+    against the plain forms and poisons any disagreement)."""
+    acc1 = rand_torus(rng, (1, 2, N))
+    acc = np.repeat(acc1, 2 * N, axis=0)
+    bara = np.arange(2 * N, dtype=np.int32)
+    rot, dig = T().rotate_decompose(FP, acc, bara)
+    for a in range(2 * N):
+        for q in range(2):
+            want = O.poly_mul_xai_minus_one(a, acc1[0, q])
+            assert np.array_equal(rot[a, q], want), (a, q)
+            if a % 64 == 0:
+                wd = O.decompose(want, P.l, P.Bgbit)
+                assert np.array_equal(dig[a, q * P.l:(q + 1) * P.l], wd), (a, q)
+
+
 def test_extract_bit_exact(rng):
     acc = rand_torus(rng, (5, 2, N))
     u = T().extract(FP, acc)
