@@ -17,6 +17,8 @@
 // The output stays in this "lane order" (for R1 = 8 it is natural order: lane L, register r
 // holds F_{L + 64 r}); the key is stored in the same order, so the pointwise product needs no
 // permutation, and the inverse runs the passes backwards from lane order to natural j order.
+// The split-FP32 kernel uses the decimation-in-time form of the same factorisation instead
+// (fft512_dit: twiddles on the butterfly inputs, fused into them, one code for both directions).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
