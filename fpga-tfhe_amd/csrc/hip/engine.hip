@@ -1257,7 +1257,7 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
                 const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                 v[aa] = w2_prep(p0, p1, rw_own[aa], wave, aa == 0);
             }
-            fft_fwd<R1, double, false, false, false>(v, tw, sx_own, lane);
+            fft_fwd<R1, double, false, false>(v, tw, sx_own, lane);
             const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
 #pragma unroll
             for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64]);
@@ -1269,8 +1269,8 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
         }
     }
     asm volatile("" ::"v"(sink));
-    fft_inv<R1, double, false, false, false>(o0, tw, sx_own, lane);
-    fft_inv<R1, double, false, false, false>(o1, tw, sx_own, lane);
+    fft_inv<R1, double, false, false>(o0, tw, sx_own, lane);
+    fft_inv<R1, double, false, false>(o1, tw, sx_own, lane);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {

@@ -295,13 +295,12 @@ __device__ __forceinline__ void exchange(C2<T>* v, void* buf, WR wr, RD rd) {
 // Forward transform of one wavefront-owned polynomial.  In: v[a] = z_{64a + lane} (already
 // multiplied by the a-dependent twist constant when the twist is folded into tw.t1).  Out: v in
 // lane order.  buf: xbuf_elems<M>() elements of C2<T> (of T when HALF) private to the wavefront.
-// UNIT0: tw.t1[0] is 1 (plain pass-1 twiddles, no twist folded in), so its multiply is skipped.
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false, bool FUSE = true, typename TW>
+template <int R1, typename T, bool HALF = false, bool FUSE = true, typename TW>
 __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
     dftR<R1, +1, T, FUSE>(v);
 #pragma unroll
-    for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw.t1[k1]);
     const int c = lane & 7, k1lo = lane >> 3;
     if constexpr (!HALF) {
         C2<T>* buf = (C2<T>*)vbuf;
@@ -353,7 +352,7 @@ __device__ __forceinline__ void fft_fwd(C2<T>* v, const TW& tw, void* vbuf, int 
 
 // Inverse of fft_fwd without the 1/M scale and without the a-dependent untwist constants:
 // in: v in lane order; out: v[a] = M * z_{64a+lane} * (conj of the forward a-constant).
-template <int R1, typename T, bool HALF = false, bool UNIT0 = false, bool FUSE = true, typename TW>
+template <int R1, typename T, bool HALF = false, bool FUSE = true, typename TW>
 __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int lane) {
     constexpr int G = R1 / 8;
     // conj(t2) on the inputs of an 8-point inverse DFT; FUSE folds each twiddle into a butterfly
@@ -407,10 +406,10 @@ __device__ __forceinline__ void fft_inv(C2<T>* v, const TW& tw, void* vbuf, int 
                            [&](int k1) { return x1_addr(k1, k1lo, c); });
     }
     if constexpr (FUSE && R1 == 8) {
-        dft8_tw<-1, true, UNIT0>(v, tw.t1);
+        dft8_tw<-1, true, false>(v, tw.t1);
     } else {
 #pragma unroll
-        for (int k1 = UNIT0 ? 1 : 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
+        for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw.t1[k1]);
         dftR<R1, -1, T, FUSE>(v);
     }
 }
