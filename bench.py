@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--audit-gates", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-reps", type=int, default=3,
+                    help="timed calls of the host-array entry point (PCIe-inclusive rate; 0: skip)")
     ap.add_argument("--prec", choices=["both", "split32", "fp64"], default="both")
     ap.add_argument("--adders", type=int, default=4096, help="32-bit adders per circuit run (0: skip)")
     ap.add_argument("--n2048-gates", type=int, default=8192,
@@ -294,6 +296,20 @@ def main():
         log(f"[bench] {name}: {r}")
 
     head = "split32" if "split32" in res else "fp64"
+    # the same batch through the host-array entry point (tfhe_gpu_gate_batch): the ciphertexts are
+    # copied in from pageable host memory and the results copied back inside each timed call
+    host = None
+    if args.host_reps > 0:
+        out_h = dk.gate("NAND", ca, cb, prec=precs[head])
+        t0 = time.perf_counter()
+        for _ in range(args.host_reps):
+            out_h = dk.gate("NAND", ca, cb, prec=precs[head])
+        sec = (time.perf_counter() - t0) / args.host_reps
+        host = {"gates_per_s": round(B / sec, 1), "ms_per_call": round(sec * 1e3, 3),
+                "bytes_in": int(ca.nbytes + cb.nbytes), "bytes_out": int(out_h.nbytes),
+                "correct": bool(np.array_equal(ks.decrypt(out_h), want)),
+                "entry": "tfhe_gpu_gate_batch (pageable host arrays, copies inside the timed call)"}
+        log(f"[bench] host-buffer {head}: {host}")
     aud = {}
     if args.audit_gates > 0:
         for name, prec in precs.items():
@@ -343,6 +359,7 @@ def main():
                        "correct": v["correct"]} for kk, v in res.items()},
         "split32_over_fp64": (round(res["split32"]["gates_per_s"] / res["fp64"]["gates_per_s"], 3)
                               if len(res) == 2 else None),
+        "host_buffers": host,
         "audit": aud or None,
         "split32_extern_product_error": prec_rep,
         "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),
