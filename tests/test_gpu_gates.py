@@ -63,6 +63,52 @@ def test_empty_and_ragged_batches(gpu_env, rng):
         assert np.array_equal(ks.decrypt(out), bits[0] & bits[1]), cnt
 
 
+def test_full_batch_16384_both_paths(gpu_env, rng):
+    """The headline workload's size (16384 NANDs in one launch, device-resident ciphertexts):
+    every output decrypts to the plaintext NAND on both paths, the host-array entry point returns
+    the device path's FP64 outputs bit for bit, and 48 gates spread over the batch (both ends
+    included) are bit-exact with the oracle on the FP64 path."""
+    ks, dk, keys = gpu_env
+    cnt = 16384
+    bits = rng.integers(0, 2, (2, cnt)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    want = 1 - (bits[0] & bits[1])
+    d_a, d_b = T().DeviceBuffer.from_array(ca), T().DeviceBuffer.from_array(cb)
+    d_out = T().DeviceBuffer(ca.nbytes)
+    try:
+        outs = {}
+        for prec in (0, 1):
+            dk.gate_dev("NAND", d_out.ptr, d_a.ptr, d_b.ptr, None, cnt, prec=prec)
+            outs[prec] = d_out.to_array(ca.shape)
+            assert np.array_equal(ks.decrypt(outs[prec]), want), prec
+    finally:
+        for b in (d_a, d_b, d_out):
+            b.free()
+    assert np.array_equal(dk.gate("NAND", ca, cb, prec=0), outs[0])
+    idx = np.unique(np.concatenate([[0, 1, cnt - 2, cnt - 1], rng.choice(cnt, 44, replace=False)]))
+    ref, _ = O.gate_batch(keys, "NAND", ca[idx], cb[idx])
+    assert np.array_equal(outs[0][idx], ref)
+
+
+def test_host_array_mux_batch_matches_device_path(gpu_env, rng):
+    """A large, ragged host-array MUX batch (8269 gates, split-FP32): outputs decrypt to the
+    plaintext MUX and are bit-identical to the device-resident entry point's."""
+    ks, dk, _ = gpu_env
+    cnt = 2 * 4096 + 77
+    bits = rng.integers(0, 2, (3, cnt)).astype(np.int32)
+    ca, cb, cc = (ks.encrypt(bits[i]) for i in range(3))
+    out = dk.gate("MUX", ca, cb, cc, prec=1)
+    assert np.array_equal(ks.decrypt(out), np.where(bits[0] == 1, bits[1], bits[2]))
+    bufs = [T().DeviceBuffer.from_array(x) for x in (ca, cb, cc)]
+    d_out = T().DeviceBuffer(ca.nbytes)
+    try:
+        dk.gate_dev("MUX", d_out.ptr, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, cnt, prec=1)
+        assert np.array_equal(d_out.to_array(ca.shape), out)
+    finally:
+        for b in bufs + [d_out]:
+            b.free()
+
+
 def test_chained_gates_stay_correct(gpu_env, rng):
     """Outputs feed the next level (bootstrapped noise, not fresh noise)."""
     ks, dk, _ = gpu_env
