@@ -88,7 +88,9 @@ void free_twiddles(DevTwiddles& tw);
 
 bool params_supported(const DevParams& p);
 // FP64 bootstraps of the N=1024 set: key-sharing workgroups (true, default) or one per workgroup
-void set_boot64_shared(bool on);
+// FP64 launch form when a launch names none: 0 one bootstrap per workgroup, 1 key-sharing
+// workgroups, 2 by launch size
+void set_boot64_shared(int mode);
 hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& L,
                             hipStream_t st);
 // KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.  Output g goes to out[g], or,

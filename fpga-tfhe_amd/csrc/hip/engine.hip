@@ -1490,9 +1490,25 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
     }
 }
 
-// FP64 bootstrap form: workgroups sharing key rows (default) or one bootstrap per workgroup
-static bool g_boot64_shared = true;
-void set_boot64_shared(bool on) { g_boot64_shared = on; }
+// FP64 bootstrap form when a launch does not name one: 0 one bootstrap per workgroup, 1 workgroups
+// sharing key rows, 2 (default) by launch size (boot64_single_round)
+static int g_boot64_shared = 2;
+void set_boot64_shared(int mode) { g_boot64_shared = mode; }
+
+// Bootstraps the GPU runs at once in the one-per-workgroup FP64 form: its 35 KB of LDS fits four
+// workgroups per CU.  A launch of at most that many finishes in one round of that form, which
+// beats the key-sharing form's round (same step count, two wavefronts per SIMD in lockstep).
+static int boot64_single_round() {
+    static int cap = 0;
+    if (!cap) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cap = 4 * cus;
+    }
+    return cap;
+}
 
 template <int N, int L, int BG>
 static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
@@ -1509,7 +1525,9 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.t1 = tw.t1_fold64;
         A.t2 = tw.t2_64;
         if constexpr (N == 1024) {
-            const bool shared = Lc.form == 1 || (Lc.form == 0 && g_boot64_shared);
+            const bool shared = Lc.form == 1 ||
+                                (Lc.form == 0 && (g_boot64_shared == 1 ||
+                                                  (g_boot64_shared == 2 && Lc.count > boot64_single_round())));
             if (Lc.mode != 2 && key.p.n <= B64W_MAXN && shared) {
                 const int blocks = (Lc.count + B64W_GATES - 1) / B64W_GATES;
                 hipLaunchKernelGGL((k_boot64w<N, L, BG>), dim3(blocks), dim3(64 * B64W_GATES), 0, st, A);

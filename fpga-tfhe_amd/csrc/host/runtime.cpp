@@ -409,8 +409,8 @@ extern "C" int tfhe_gpu_set_split_bits(int32_t bits) {
 }
 
 extern "C" int tfhe_gpu_set_fp64_shared_rows(int32_t on) {
-    if (on != 0 && on != 1) return arg_fail("fp64 shared-rows flag must be 0 or 1");
-    set_boot64_shared(on != 0);
+    if (on < 0 || on > 2) return arg_fail("fp64 shared-rows flag must be 0, 1 or 2");
+    set_boot64_shared(on);
     return TFHE_OK;
 }
 

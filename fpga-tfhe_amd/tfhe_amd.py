@@ -257,7 +257,9 @@ def bk_from_lagrange(flat_params, lagrange):
 
 
 def set_fp64_shared_rows(on):
-    _ck(lib().tfhe_gpu_set_fp64_shared_rows(1 if on else 0), "set_fp64_shared_rows")
+    """FP64 launch form: True (key-sharing workgroups), False (one bootstrap per workgroup) or
+    "auto" (the default: by launch size)."""
+    _ck(lib().tfhe_gpu_set_fp64_shared_rows(2 if on == "auto" else (1 if on else 0)), "set_fp64_shared_rows")
 
 
 class DeviceKey:

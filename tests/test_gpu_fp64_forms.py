@@ -21,7 +21,7 @@ def T():
 @pytest.fixture
 def forms():
     yield
-    T().set_fp64_shared_rows(True)
+    T().set_fp64_shared_rows("auto")
 
 
 @pytest.mark.parametrize("shared", [True, False])
@@ -42,19 +42,22 @@ def test_blind_rotate_fp64_forms_match_oracle(gpu_env, forms, shared):
         assert np.array_equal(got[g], O.blind_rotate(keys, acc[g], bara[g])), g
 
 
-def test_fp64_forms_agree_on_gates(gpu_env, forms):
+@pytest.mark.parametrize("cnt", [8 * 5 + 3, 1100])
+def test_fp64_forms_agree_on_gates(gpu_env, forms, cnt):
+    """Both forms forced, and the default choice by size (one per workgroup up to 4 per CU, key
+    sharing above), give identical outputs."""
     ks, dk, keys = gpu_env
     rng = np.random.default_rng(78)
-    cnt = 8 * 5 + 3
     bits = rng.integers(0, 2, (3, cnt)).astype(np.int32)
     ca, cb, cc = (ks.encrypt(bits[i]) for i in range(3))
     out = {}
-    for shared in (True, False):
+    for shared in (True, False, "auto"):
         T().set_fp64_shared_rows(shared)
         print(f"[fp64 forms] shared={shared} NAND/MUX count={cnt}", flush=True)
         out[shared] = (dk.gate("NAND", ca, cb, prec=0), dk.gate("MUX", ca, cb, cc, prec=0))
     for k in range(2):
         assert np.array_equal(out[True][k], out[False][k])
+        assert np.array_equal(out["auto"][k], out[False][k])
     assert np.array_equal(ks.decrypt(out[True][0]), 1 - (bits[0] & bits[1]))
     assert np.array_equal(ks.decrypt(out[True][1]), np.where(bits[0] == 1, bits[1], bits[2]))
     for i in (0, cnt - 1):
@@ -63,5 +66,6 @@ def test_fp64_forms_agree_on_gates(gpu_env, forms):
 
 def test_fp64_shared_rows_flag_rejects_bad_values():
     import tfhe_amd
-    with pytest.raises(tfhe_amd.TfheError):
-        tfhe_amd._ck(tfhe_amd.lib().tfhe_gpu_set_fp64_shared_rows(2), "set_fp64_shared_rows")
+    for bad in (-1, 3):
+        with pytest.raises(tfhe_amd.TfheError):
+            tfhe_amd._ck(tfhe_amd.lib().tfhe_gpu_set_fp64_shared_rows(bad), "set_fp64_shared_rows")
