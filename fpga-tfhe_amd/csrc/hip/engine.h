@@ -54,7 +54,7 @@ struct LinDesc {
 //   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out optional.
 //   mode 3: as mode 0, with the gate inputs of job g taken from segs[g / seg_jobs].
 // form (FP64, N=1024): 0 = the process-wide choice, 1 = key-sharing workgroups, 2 = one bootstrap
-// per workgroup.
+// per workgroup, 3 = one bootstrap per workgroup of 2l wavefronts (latency form; modes 0, 1, 3).
 struct BootLaunch {
     int mode = 0;
     int form = 0;

@@ -280,6 +280,101 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
 }
 
 // ---------------------------------------------------------------------------------------------
+// FP64 path, latency form: one bootstrap per workgroup of 2l wavefronts.  In each step wavefront
+// w = (q, lvl) rotates polynomial q of the accumulator, transforms its level-lvl digits and
+// multiplies them by key row w, leaving the two partial products in LDS.  After a barrier,
+// wavefronts 0 and 1 each sum the 2l partials of one output polynomial in a fixed order, invert it
+// and add it into the accumulator.  A step's critical path is one forward and one inverse transform
+// instead of eight transforms in sequence.  Each wavefront loads its key row for the next step
+// into registers as soon as its products are written.  For launches too small to fill the GPU
+// (single gates); its outputs equal the other FP64 forms' bit for bit, since each is the exact
+// integer that the final rounding recovers.
+// ---------------------------------------------------------------------------------------------
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64 * 2 * L) k_boot64lat(BootKArgs A) {
+    constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1, KPL = 2 * L;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    constexpr size_t ROW = (size_t)2 * M, ROWSET = (size_t)KPL * ROW;
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ int16_t s_bara[kMaxLweN];
+    __shared__ double s_x[KPL][xbuf_elems<M>()];   // half-width exchange buffer of each wavefront
+    __shared__ C2<double> s_part[KPL][2][M];        // partial products [key row][output polynomial]
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = blockIdx.x;
+    if (w == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    __syncthreads();
+
+    LaneTw<double, R1> tw;
+    load_lane_tw(tw, (const C2<double>*)A.t1, (const C2<double>*)A.t2, lane);
+    C2<double> wa[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) wa[r] = ((const C2<double>*)A.wa64)[r];
+    const int q = w / L, lvl = w % L;
+    const C2<double>* krow = (const C2<double>*)A.bk + (size_t)w * ROW + lane;
+    C2<double> k0[R1], k1[R1];
+    auto load_row = [&](int i) {
+        const C2<double>* p = krow + (size_t)i * ROWSET;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) { k0[r] = p[r * 64]; k1[r] = p[M + r * 64]; }
+    };
+    const int n = A.n;
+    int loaded = -1;
+    for (int i = 0; i < n; ++i) {
+        // the same value for every wavefront: the whole workgroup skips a zero rotation
+        // (tfhe_blindRotate_FFT.cpp:114)
+        const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+        if (a == 0) continue;
+        if (loaded != i) load_row(i);
+        {
+            uint32_t u[NC];
+            const uint32_t base4 = rot_base4(lane, a);
+#pragma unroll
+            for (int r = 0; r < NC; ++r)
+                u[r] = rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, lane + 64 * r, OFF) ^ OFF;
+            C2<double> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                v[aa] = C2<double>{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + R1], lvl)};
+                if (aa) v[aa] = cmul(v[aa], wa[aa]);
+            }
+            fft_fwd<R1, double, true>(v, tw, s_x[w], lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                s_part[w][0][r * 64 + lane] = cmul(v[r], k0[r]);
+                s_part[w][1][r * 64 + lane] = cmul(v[r], k1[r]);
+            }
+        }
+        if (i + 1 < n) load_row(i + 1);
+        loaded = i + 1;
+        // the partials are complete, and every wavefront has read the accumulator for its rotation
+        __syncthreads();
+        if (w < 2) {
+            C2<double> o[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                o[r] = s_part[0][w][r * 64 + lane];
+#pragma unroll
+                for (int t = 1; t < KPL; ++t) o[r] = o[r] + s_part[t][w][r * 64 + lane];
+            }
+            fft_inv<R1, double, true>(o, tw, s_x[w], lane);
+            const double sc = 1.0 / (double)M;
+            int32_t* acc = s_acc + w * N;
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const int j = lane + 64 * aa;
+                const C2<double> z = aa ? cmulc(o[aa], wa[aa]) : o[aa];
+                acc[j] = (int32_t)((uint32_t)acc[j] + round_mod32(z.x * sc));
+                acc[j + M] = (int32_t)((uint32_t)acc[j + M] + round_mod32(z.y * sc));
+            }
+        }
+        // the accumulator is updated and the partial buffers are free
+        __syncthreads();
+    }
+    if (w == 0) boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// ---------------------------------------------------------------------------------------------
 // FP64 path, key-sharing form.  A workgroup of B64W_GATES wavefronts runs that many bootstraps in
 // lockstep over the key: every key row is brought into LDS once per workgroup, each wavefront
 // loading its share with direct global->LDS loads, instead of once per bootstrap.  Per-bootstrap
@@ -693,6 +788,61 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
         for (int c = 0; c < C; ++c)
             if (c0 + c <= n) o[c0 + c] = (int32_t)acc[c];
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Keyswitch of a few ciphertexts (latency).  The tiled kernel above walks all KN input indices in
+// sequence in each of its NP/C workgroups, which for a single gate is most of a millisecond.  Here
+// the input indices are spread over KS_SLICES workgroups per ciphertext and 64-coefficient chunk:
+// each subtracts its slice's selected digit-pair rows straight from the key (the digit pair is
+// uniform in the workgroup, the read coalesced across coefficients), and adds its partial sum into
+// the output with a 32-bit atomic.  Integer addition wraps mod 2^32 in any order, so the result
+// is bit-identical to the tiled kernel's.  k_ks_clear zeroes the outputs first (a kernel rather
+// than a memset, so that a captured graph records no memset node).
+// ---------------------------------------------------------------------------------------------
+constexpr int KS_SLICES = 32;
+constexpr int KS_SMALL_MAX = 64;   // ciphertexts per launch up to which the sliced form is used
+
+__device__ __forceinline__ int32_t* ks_out_row(int32_t* out, int32_t* const* out_tab, int out_seg, int g, int n) {
+    return out_tab ? out_tab[g / out_seg] + (size_t)(g % out_seg) * (n + 1) : out + (size_t)g * (n + 1);
+}
+
+__global__ void __launch_bounds__(256) k_ks_clear(int32_t* out, int n, int32_t* const* out_tab, int out_seg) {
+    int32_t* o = ks_out_row(out, out_tab, out_seg, blockIdx.x, n);
+    for (int c = threadIdx.x; c <= n; c += 256) o[c] = 0;
+}
+
+template <int TKS, int KN, int NP>
+__global__ void __launch_bounds__(64) k_keyswitch_sliced(const int32_t* __restrict__ u1,
+                                                         const int32_t* __restrict__ u2, int32_t add_b,
+                                                         const int32_t* __restrict__ ksk, int32_t* out, int n,
+                                                         int32_t* const* out_tab, int out_seg) {
+    static_assert(KN % KS_SLICES == 0 && NP % 64 == 0 && TKS % 2 == 0, "sliced keyswitch tiling");
+    constexpr int TP = TKS / 2, RPI = TP * KS_PAIR_ROWS, ISL = KN / KS_SLICES;
+    constexpr size_t UST = (size_t)KN + 1;
+    const int c = blockIdx.x * 64 + threadIdx.x;   // < NP: the key rows are padded to NP
+    const int s = blockIdx.y;
+    const int g = blockIdx.z;
+    const uint32_t prec_off = 1u << (32 - (1 + 2 * TKS));
+    const int32_t* x1 = u1 + (size_t)g * UST;
+    const int32_t* x2 = u2 ? u2 + (size_t)g * UST : nullptr;
+    uint32_t acc = 0;
+    for (int i = s * ISL; i < (s + 1) * ISL; ++i) {
+        uint32_t abar = (uint32_t)x1[i] + prec_off;
+        if (x2) abar += (uint32_t)x2[i];
+        const int32_t* kr = ksk + (size_t)i * RPI * NP + c;
+#pragma unroll
+        for (int jp = 0; jp < TP; ++jp) {
+            const uint32_t d = (abar >> (28 - 4 * jp)) & 15u;
+            if (d) acc -= (uint32_t)kr[(size_t)(jp * KS_PAIR_ROWS + (int)d - 1) * NP];
+        }
+    }
+    if (c > n) return;
+    if (s == 0 && c == n) {
+        acc += (uint32_t)x1[KN] + (uint32_t)add_b;
+        if (x2) acc += (uint32_t)x2[KN];
+    }
+    atomicAdd(ks_out_row(out, out_tab, out_seg, g, n) + c, (int32_t)acc);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1491,24 +1641,31 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
 }
 
 // FP64 bootstrap form when a launch does not name one: 0 one bootstrap per workgroup, 1 workgroups
-// sharing key rows, 2 (default) by launch size (boot64_single_round)
+// sharing key rows, 2 (default) by launch size (boot64_single_round, boot64_latency_max)
 static int g_boot64_shared = 2;
 void set_boot64_shared(int mode) { g_boot64_shared = mode; }
+
+static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// Launches of at most this many bootstraps take the latency form (k_boot64lat): with its 133 KB of
+// LDS one workgroup runs per CU, and one round of its short steps beats the one round of the
+// one-per-workgroup form (4.4 against 7.0 ms for a gate); two rounds do not (8.5 against 8.0 ms
+// for 512 gates on 256 CUs).
+static int boot64_latency_max() { return device_cus(); }
 
 // Bootstraps the GPU runs at once in the one-per-workgroup FP64 form: its 35 KB of LDS fits four
 // workgroups per CU.  A launch of at most that many finishes in one round of that form, which
 // beats the key-sharing form's round (same step count, two wavefronts per SIMD in lockstep).
-static int boot64_single_round() {
-    static int cap = 0;
-    if (!cap) {
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        cap = 4 * cus;
-    }
-    return cap;
-}
+static int boot64_single_round() { return 4 * device_cus(); }
 
 template <int N, int L, int BG>
 static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
@@ -1525,6 +1682,11 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.t1 = tw.t1_fold64;
         A.t2 = tw.t2_64;
         if constexpr (N == 1024) {
+            if (Lc.mode != 2 && key.p.n <= kMaxLweN &&
+                (Lc.form == 3 || (Lc.form == 0 && g_boot64_shared == 2 && Lc.count <= boot64_latency_max()))) {
+                hipLaunchKernelGGL((k_boot64lat<N, L, BG>), dim3(Lc.count), dim3(64 * 2 * L), 0, st, A);
+                return hipGetLastError();
+            }
             const bool shared = Lc.form == 1 ||
                                 (Lc.form == 0 && (g_boot64_shared == 1 ||
                                                   (g_boot64_shared == 2 && Lc.count > boot64_single_round())));
@@ -1580,6 +1742,12 @@ static hipError_t launch_ks_t(const DevKeyView& key, int32_t* out, const int32_t
                               int32_t add_b, int count, hipStream_t st, int32_t* const* out_tab, int out_seg) {
     constexpr int C = 32;
     if (key.ksk_np != NP || key.p.N != KN) return hipErrorInvalidValue;
+    if (count <= KS_SMALL_MAX) {
+        hipLaunchKernelGGL(k_ks_clear, dim3(count), dim3(256), 0, st, out, key.p.n, out_tab, out_seg);
+        hipLaunchKernelGGL((k_keyswitch_sliced<TKS, KN, NP>), dim3(NP / 64, KS_SLICES, count), dim3(64), 0, st, u1,
+                           u2, add_b, key.ksk, out, key.p.n, out_tab, out_seg);
+        return hipGetLastError();
+    }
     dim3 grid(NP / C, (count + KS_GATES - 1) / KS_GATES);
     hipLaunchKernelGGL((k_keyswitch<TKS, KN, NP, C>), grid, dim3(KS_GATES), 0, st, u1, u2, add_b, key.ksk, out, count,
                        key.p.n, out_tab, out_seg);

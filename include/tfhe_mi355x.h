@@ -346,8 +346,9 @@ int tfhe_gpu_set_default_precision(int32_t prec);
 /* launch form of the FP64 bootstrap (no reference counterpart: a tuning knob of this engine).
  * 1: workgroups of 8 bootstraps share each key row through LDS, 2 wavefronts per SIMD;
  * 0: one bootstrap per workgroup with a private LDS key-row buffer;
- * 2 (default): form 1 for launches of more bootstraps than the GPU runs at once in form 0
- *    (4 per CU), form 0 for smaller ones.  Results are identical in every form. */
+ * 2 (default): by launch size: up to 1 bootstrap per CU, a latency form that spreads each
+ *    bootstrap over six wavefronts; up to 4 per CU (what form 0 runs at once), form 0; above
+ *    that, form 1.  Results are identical in every form. */
 int tfhe_gpu_set_fp64_shared_rows(int32_t on);
 
 /* flat views of host key sets made by new_random_gate_bootstrapping_secret_keyset (any output may

@@ -1,5 +1,6 @@
-"""GPU: the two launch forms of the FP64 bootstrap (key rows shared by a workgroup of 8 bootstraps,
-or one bootstrap per workgroup) against the oracle and against each other.  Batch sizes that are
+"""GPU: the launch forms of the FP64 bootstrap (key rows shared by a workgroup of 8 bootstraps, one
+bootstrap per workgroup, and the latency form that spreads one bootstrap over six wavefronts)
+against the oracle and against each other.  Batch sizes that are
 not multiples of 8 leave idle wavefronts in the last workgroup; zero rotations make some
 bootstraps of a workgroup skip steps that the others run."""
 import numpy as np
@@ -24,8 +25,10 @@ def forms():
     T().set_fp64_shared_rows("auto")
 
 
-@pytest.mark.parametrize("shared", [True, False])
+@pytest.mark.parametrize("shared", [True, False, "auto"])
 def test_blind_rotate_fp64_forms_match_oracle(gpu_env, forms, shared):
+    """Key-sharing workgroups (True), one bootstrap per workgroup (False), and the default choice
+    for a launch this small: the latency form, six wavefronts per bootstrap."""
     _, dk, keys = gpu_env
     T().set_fp64_shared_rows(shared)
     rng = np.random.default_rng(77)
