@@ -283,8 +283,10 @@ int enqueue_all(tfhe_circuit* c, DevTwiddles* tw) {
     return TFHE_OK;
 }
 
-// Times both FP64 forms of each level width not seen before.  The levels run on whatever the wire
-// storage holds (only the timing is used); the real evaluation follows.
+// Times the FP64 launch forms of each level width not seen before: key-sharing workgroups, one
+// bootstrap per workgroup, and for levels of at most 1024 bootstraps the latency form (one
+// bootstrap per workgroup of six wavefronts).  The levels run on whatever the wire storage holds
+// (only the timing is used); the real evaluation follows.
 int autotune(tfhe_circuit* c, DevTwiddles* tw) {
     const DevKeyView kv = key_view(c->key);
     hipStream_t st = lib_stream();
@@ -298,7 +300,8 @@ int autotune(tfhe_circuit* c, DevTwiddles* tw) {
         if (it == tuned_forms().end()) {
             float best = 0.f;
             int best_form = 1;
-            for (int form : {1, 2}) {
+            for (int form : {1, 2, 3}) {
+                if (form == 3 && jobs > 1024) continue;
                 hipEvent_t e0, e1;
                 if (hipEventCreate(&e0) != hipSuccess) return lib_hip_fail("event", hipErrorUnknown);
                 if (hipEventCreate(&e1) != hipSuccess) return lib_hip_fail("event", hipErrorUnknown);

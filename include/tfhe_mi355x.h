@@ -448,9 +448,10 @@ int tfhe_circuit_level_of(tfhe_circuit* c, int32_t wire, int32_t* level);
 /* Evaluates `batch` independent instances.  d_in: device [n_inputs][batch][n+1], d_out: device
  * [n_outputs][batch][n+1].  Asynchronous on the library stream (tfhe_gpu_sync waits).  flags:
  * TFHE_CIRCUIT_GRAPH captures the launch sequence into a HIP graph on the first run of a binding
- * (key, precision, batch, d_in, d_out) and replays it afterwards; TFHE_CIRCUIT_AUTOTUNE times both
- * FP64 launch forms for every level width not seen before and keeps the faster.  Runs of one
- * circuit must not overlap (its wire storage is shared). */
+ * (key, precision, batch, d_in, d_out) and replays it afterwards; TFHE_CIRCUIT_AUTOTUNE times the
+ * FP64 launch forms (key-sharing, one bootstrap per workgroup and, up to 1024 bootstraps, the
+ * latency form) for every level width not seen before and keeps the fastest.  Runs of one circuit
+ * must not overlap (its wire storage is shared). */
 int tfhe_circuit_run(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec, const int32_t* d_in, int32_t* d_out,
                      int32_t batch, int32_t flags);
 /* One level of independent gates, synchronous: gate i (TFHE_GATE_*) reads the device arrays a[i],
