@@ -310,6 +310,19 @@ def main():
                 "correct": bool(np.array_equal(ks.decrypt(out_h), want)),
                 "entry": "tfhe_gpu_gate_batch (pageable host arrays, copies inside the timed call)"}
         log(f"[bench] host-buffer {head}: {host}")
+    # one gate per call through the same entry point (the upstream single-gate API's shape): the
+    # small-launch forms (FP64 latency form, sliced keyswitch) and the copies, end to end
+    single = {}
+    if args.host_reps > 0:
+        for name, prec in precs.items():
+            dk.gate("NAND", ca[:1], cb[:1], prec=prec)
+            t0 = time.perf_counter()
+            for _ in range(args.host_reps):
+                o1 = dk.gate("NAND", ca[:1], cb[:1], prec=prec)
+            sec = (time.perf_counter() - t0) / args.host_reps
+            single[name] = {"ms_per_gate": round(sec * 1e3, 3),
+                            "correct": bool(ks.decrypt(o1)[0] == want[0])}
+        log(f"[bench] single gate: {single}")
     aud = {}
     if args.audit_gates > 0:
         for name, prec in precs.items():
@@ -360,6 +373,7 @@ def main():
         "split32_over_fp64": (round(res["split32"]["gates_per_s"] / res["fp64"]["gates_per_s"], 3)
                               if len(res) == 2 else None),
         "host_buffers": host,
+        "single_gate_latency": single or None,
         "audit": aud or None,
         "split32_extern_product_error": prec_rep,
         "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),

@@ -801,7 +801,9 @@ __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restric
 // than a memset, so that a captured graph records no memset node).
 // ---------------------------------------------------------------------------------------------
 constexpr int KS_SLICES = 32;
-constexpr int KS_SMALL_MAX = 64;   // ciphertexts per launch up to which the sliced form is used
+// ciphertexts per launch up to which the sliced form is used: its time grows with the count (35 us
+// for 1, 120 us for 64), the tiled kernel's does not below 2048 (1.2 ms)
+constexpr int KS_SMALL_MAX = 512;
 
 __device__ __forceinline__ int32_t* ks_out_row(int32_t* out, int32_t* const* out_tab, int out_seg, int g, int n) {
     return out_tab ? out_tab[g / out_seg] + (size_t)(g % out_seg) * (n + 1) : out + (size_t)g * (n + 1);
