@@ -662,6 +662,103 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Split-FP32 path, latency form: one bootstrap per workgroup of 2l wavefronts, as k_boot64lat.
+// Wavefront w = (q, lvl) transforms its digits and leaves four partial products in LDS (hi and lo
+// key halves of both output polynomials); after a barrier, wavefronts 0-3 each sum the 2l partials
+// of one (output polynomial, half), invert it, untwist and round it in FP64, and add it into the
+// accumulator with an LDS integer atomic (the hi half shifted by the split), exact mod 2^32 in
+// either order.
+// ---------------------------------------------------------------------------------------------
+template <int N, int L, int BG>
+__global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
+    constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1, KPL = 2 * L;
+    static_assert(M == 512 && KPL >= 4, "the 512-point decimation-in-time transform; four output halves");
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    constexpr uint32_t ROW_B = (uint32_t)(2 * M * sizeof(float4)), ROWSET_B = KPL * ROW_B;
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ int16_t s_bara[kMaxLweN];
+    __shared__ C2<float> s_x[KPL][xbuf_elems<M>()];
+    __shared__ C2<float> s_part[KPL][4][M];   // [key row][(output polynomial j, half h) = 2j + h]
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = blockIdx.x;
+    if (w == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    __syncthreads();
+
+    DitTw<float> tw;
+    load_dit_tw(tw, (const C2<float>*)A.t2, (const C2<float>*)A.t1, lane);
+    C2<float> tws[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
+    const C2<double>* untw = (const C2<double>*)A.untw64;
+    const __amdgpu_buffer_rsrc_t key = key_rsrc(A.bk, (uint32_t)A.n * ROWSET_B);
+    const int sb = A.split_bits;
+    const int q = w / L, lvl = w % L;
+    float4 k0[R1], k1[R1];
+    auto load_row = [&](int i) {
+        const uint32_t ro = (uint32_t)i * ROWSET_B + (uint32_t)w * ROW_B;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            k0[r] = key_load(key, (uint32_t)lane * sizeof(float4), ro + r * 64 * sizeof(float4));
+            k1[r] = key_load(key, (uint32_t)lane * sizeof(float4), ro + (M + r * 64) * sizeof(float4));
+        }
+    };
+    const int n = A.n;
+    int loaded = -1;
+    for (int i = 0; i < n; ++i) {
+        const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+        if (a == 0) continue;
+        if (loaded != i) load_row(i);
+        {
+            uint32_t u[NC];
+            const uint32_t base4 = rot_base4(lane, a);
+#pragma unroll
+            for (int r = 0; r < NC; ++r)
+                u[r] = rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, lane + 64 * r, OFF) ^ OFF;
+            C2<float> v[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa)
+                v[aa] = C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)};
+            fft512_dit<+1, true>(v, tw, tws, s_x[w], lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                s_part[w][0][r * 64 + lane] = cmul(v[r], C2<float>{k0[r].x, k0[r].y});
+                s_part[w][1][r * 64 + lane] = cmul(v[r], C2<float>{k0[r].z, k0[r].w});
+                s_part[w][2][r * 64 + lane] = cmul(v[r], C2<float>{k1[r].x, k1[r].y});
+                s_part[w][3][r * 64 + lane] = cmul(v[r], C2<float>{k1[r].z, k1[r].w});
+            }
+        }
+        if (i + 1 < n) load_row(i + 1);
+        loaded = i + 1;
+        __syncthreads();
+        if (w < 4) {
+            const int j = w >> 1, hi = !(w & 1);
+            C2<float> o[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                o[r] = s_part[0][w][r * 64 + lane];
+#pragma unroll
+                for (int t = 1; t < KPL; ++t) o[r] = o[r] + s_part[t][w][r * 64 + lane];
+            }
+            fft512_dit<-1, false>(o, tw, (const C2<float>*)nullptr, s_x[w], lane);
+            int uoff = lane * R1;
+            asm volatile("" : "+v"(uoff));
+            int32_t* acc = s_acc + j * N;
+            const int sh = hi ? sb : 0;
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const int jj = lane + 64 * aa;
+                const C2<double> z = cmulc(C2<double>{(double)o[aa].x, (double)o[aa].y}, untw[uoff + aa]);
+                atomicAdd(acc + jj, (int32_t)(round_mod32(z.x) << sh));
+                atomicAdd(acc + jj + M, (int32_t)(round_mod32(z.y) << sh));
+            }
+        }
+        __syncthreads();
+    }
+    if (w == 0) boot_epilogue<N>(A, g, lane, s_acc);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Keyswitch (lwe-keyswitch-functions.cpp:114-129, :228-241), bit-exact.
 // The t base-4 digits of each input coefficient are taken in pairs: the key holds, for every
 // input index i and digit pair g, the 15 sums ks[i][2g][d1] + ks[i][2g+1][d2] over (d1, d2) != 0
@@ -1703,6 +1800,12 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.bk = key.bk32;
         A.t1 = tw.t1_plain32;   // W_512^{c L}: pass-3 input twiddles of the decimation-in-time transform
         A.t2 = tw.t2dit32;
+        if constexpr (N == 1024) {
+            if (Lc.mode != 2 && Lc.count <= boot64_latency_max()) {
+                hipLaunchKernelGGL((k_boot32lat<N, L, BG>), dim3(Lc.count), dim3(64 * 2 * L), 0, st, A);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((k_boot32<N, L, BG>), dim3(Lc.count), dim3(64), 0, st, A);
     }
     return hipGetLastError();
