@@ -1,11 +1,19 @@
 // engine.hip -- gfx950 kernels of the TFHE gate-bootstrapping path and their launchers.
 //
-// Kernels (one 64-lane workgroup = one wavefront per unit of work):
-//   k_boot64 / k_boot32  fused bootstrap: gate pre-combination, modulus switch, test vector,
-//                        n MuxRotate steps with the TLWE accumulator resident in LDS, sample
-//                        extraction.  FP64 transform path / split-FP32 transform path.
-//   k_keyswitch          bit-exact LWE keyswitch, a tile of G ciphertexts per workgroup so each
-//                        keyswitch-key row is read once per tile (coalesced over coefficients).
+// Kernels:
+//   k_boot64 / k_boot32  fused bootstrap, one wavefront per bootstrap: gate pre-combination,
+//                        modulus switch, test vector, n MuxRotate steps with the TLWE
+//                        accumulator resident in LDS, sample extraction.  FP64 transform path /
+//                        split-FP32 transform path.
+//   k_boot64w            the FP64 bootstrap for large launches: workgroups of 8 bootstraps that
+//                        share each key row through LDS.
+//   k_boot64lat / k_boot32lat  latency forms for small launches: one bootstrap spread over a
+//                        workgroup of 2l wavefronts, one per digit polynomial.
+//   k_boot2w             N = 2048: each transform split over the two wavefronts of a bootstrap.
+//   k_keyswitch          bit-exact LWE keyswitch, a tile of ciphertexts per workgroup so each
+//                        keyswitch-key row is read once per tile (coalesced over coefficients);
+//                        k_keyswitch_sliced spreads a few ciphertexts' input indices over
+//                        workgroups instead.
 //   k_keyprep            time-domain bootstrapping key -> FFT domain (FP64 or FP32 hi/lo).
 //   k_ksk_relayout       keyswitch key -> [kN][t/2][15][NP] sums of digit-pair rows.
 //   k_extract, k_rotdec, k_fft_test, k_decrypt  stage kernels for parity tests and audits.
