@@ -91,12 +91,16 @@ def test_extern_product_n2048_split32_error(env2048):
     assert e.max() < 2**18
 
 
-def test_keyswitch_n2048_bit_exact(env2048):
+@pytest.mark.parametrize("cnt", [20, 600])
+def test_keyswitch_n2048_bit_exact(env2048, cnt):
+    """The sliced kernel (up to 512 ciphertexts) and the tiled one, bit-exact with the oracle on
+    20 samples, ends included."""
     _, dk, keys = env2048
     rng = np.random.default_rng(2052)
-    u = rand_torus(rng, (20, N + 1))
+    u = rand_torus(rng, (cnt, N + 1))
     got = dk.keyswitch(u)
-    for g in range(u.shape[0]):
+    idx = sorted({0, cnt - 1} | set(rng.choice(cnt, min(cnt, 18), replace=False).tolist()))
+    for g in idx:
         assert np.array_equal(got[g], O.keyswitch(keys, u[g])), g
 
 
