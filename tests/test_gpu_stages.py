@@ -154,19 +154,22 @@ def test_blind_rotate_fp64_matches_exact_int64(gpu_env, rng):
     assert np.array_equal(got[0], want)
 
 
-def test_blind_rotate_split32_phase_error(gpu_env, rng):
+@pytest.mark.parametrize("cnt", [2, 300])
+def test_blind_rotate_split32_phase_error(gpu_env, rng, cnt):
     """After an inexact step the mask words diverge (different digits select different key-row
     combinations), so both paths are measured by the TLWE phase b - a.s against the noiseless
-    rotation X^(sum bara_i s_i) * testvector, which is what decryption sees."""
+    rotation X^(sum bara_i s_i) * testvector, which is what decryption sees.  Two accumulators run
+    the latency forms; 300 (more than one per CU) run the batched kernels, of which the first two
+    and the last are checked."""
     _, dk, keys = gpu_env
-    acc = np.stack([O.test_vector(P, b, T().MU) for b in (99, 1234)])
-    bara = rng.integers(0, 2 * N, (2, P.n)).astype(np.int32)
+    acc = np.stack([O.test_vector(P, int(b), T().MU) for b in [99, 1234] + list(rng.integers(0, 2 * N, cnt - 2))])
+    bara = rng.integers(0, 2 * N, (cnt, P.n)).astype(np.int32)
     res = {p: dk.blind_rotate(acc, bara, prec=p) for p in (0, 1)}
     rms = {}
     worst = {}
     for p in (0, 1):
         errs = []
-        for g in range(2):
+        for g in sorted({0, 1, cnt - 1}):
             rot = int((bara[g].astype(np.int64) * keys.lwe_key).sum() % (2 * N))
             ideal = O.poly_mul_xai(rot, acc[g, 1])
             ph = O.tlwe_phase(P, res[p][g], keys.tlwe_key)
