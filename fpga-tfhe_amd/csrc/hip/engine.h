@@ -28,6 +28,7 @@ struct DevTwiddles {
     void* t1w32 = nullptr;       // [2][64][8] C2<float>
     void* rw64 = nullptr;        // [2][8] C2<double>: w^{64 a (1 + 4v)}
     void* rw32 = nullptr;        // [2][8] C2<float>
+    void* rws64 = nullptr;       // [2][8] C2<double>: w^{64 a (1 + 4v)} / 1024 (split path, own untwist)
 };
 
 // One gate segment of a table-driven bootstrap launch (mode 3): jobs [s*seg_jobs, (s+1)*seg_jobs)
@@ -87,7 +88,6 @@ hipError_t build_twiddles(DevTwiddles& tw, int N);
 void free_twiddles(DevTwiddles& tw);
 
 bool params_supported(const DevParams& p);
-// FP64 bootstraps of the N=1024 set: key-sharing workgroups (true, default) or one per workgroup
 // FP64 launch form when a launch names none: 0 one bootstrap per workgroup, 1 key-sharing
 // workgroups, 2 by launch size
 void set_boot64_shared(int mode);

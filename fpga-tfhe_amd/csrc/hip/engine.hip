@@ -1254,7 +1254,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     }
     // two-wavefront form (N = 2048): pass-1 twiddles of the 512-point transform of wavefront v with
     // its lane twist w^{(1+4v) L} folded in, and its register twist constants w^{64 a (1+4v)}
-    std::vector<double> t1w(2 * 2 * 64 * 8), rw(2 * 2 * 8);
+    std::vector<double> t1w(2 * 2 * 64 * 8), rw(2 * 2 * 8), rws(2 * 2 * 8);
     std::vector<float> t1w32(t1w.size()), rw32(rw.size());
     if (N == 2048) {
         for (int v = 0; v < 2; ++v) {
@@ -1271,6 +1271,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
                 cis_pi_over_N((long long)64 * a * (1 + 4 * v), N, &re, &im);
                 rw[2 * (v * 8 + a)] = re; rw[2 * (v * 8 + a) + 1] = im;
                 rw32[2 * (v * 8 + a)] = (float)re; rw32[2 * (v * 8 + a) + 1] = (float)im;
+                rws[2 * (v * 8 + a)] = re / 1024; rws[2 * (v * 8 + a) + 1] = im / 1024;
             }
         }
     }
@@ -1294,13 +1295,14 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
         if ((e = up(&tw.t1w32, t1w32.data(), t1w32.size() * 4)) != hipSuccess) return e;
         if ((e = up(&tw.rw64, rw.data(), rw.size() * 8)) != hipSuccess) return e;
         if ((e = up(&tw.rw32, rw32.data(), rw32.size() * 4)) != hipSuccess) return e;
+        if ((e = up(&tw.rws64, rws.data(), rws.size() * 8)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 void free_twiddles(DevTwiddles& tw) {
     void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.t2dit32, tw.twist32, tw.untw64, tw.wa64,
-                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32};
+                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -1352,6 +1354,24 @@ __device__ __forceinline__ C2<double> w2_pair(C2<double> own, C2<double> other, 
     return cmulc(d, w2_c<double>());
 }
 
+// Split path: y_v[j'] before its lane twist, (P0 + hc (p1.x - p1.y, p1.x + p1.y)) w^{64a(1+4v)},
+// hc = +-cos(pi/4) the sign of wavefront v: P0 +- c P1 in two subtractions and two FMAs, then the
+// register twist (skipped for a = 0)
+__device__ __forceinline__ C2<float> w2_prep32(float p0x, float p0y, float p1x, float p1y, float hc, C2<float> rw,
+                                               bool first) {
+    const C2<float> y{__builtin_fmaf(hc, p1x - p1y, p0x), __builtin_fmaf(hc, p1x + p1y, p0y)};
+    return first ? y : cmul(y, rw);
+}
+
+// Split path: the pair this wavefront keeps from its own and the partner's inverse outputs, each
+// untwisted and scaled by 1/1024 by its own wavefront: wavefront 0 E + O, wavefront 1 (E - O) c^*
+__device__ __forceinline__ C2<double> w2_pair_own(C2<double> own, C2<double> other, int wave) {
+    if (!wave) return C2<double>{own.x + other.x, own.y + other.y};
+    const double h = 0.70710678118654752440;
+    const double dx = other.x - own.x, dy = other.y - own.y;
+    return C2<double>{h * (dx + dy), h * (dy - dx)};
+}
+
 // coefficient index (within one polynomial) of the real / imaginary part that wavefront `wave`
 // keeps for register a
 __device__ __forceinline__ int w2_coef(int wave, int lane, int a, int imag) {
@@ -1367,16 +1387,20 @@ __device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, floa
     sink += p[lane * 32] + p[W2_M * 4 + lane * 32];
 }
 
-// Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.
+// Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.  Each wavefront untwists
+// its own inverse outputs in FP64 (rws: register constants with the 1/1024) and the untwisted values
+// are exchanged, so no wavefront untwists its partner's.  sx_own / sx_other: 8 KB per wavefront,
+// the transform exchange buffer (C2<float>) and then the output exchange (C2<double>[512]).
 template <int L, int BG, bool ROT>
-__device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, const C2<float>* sx_other, int a,
+__device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const void* sx_other, int a,
                                          const float4* __restrict__ kr, const float4* __restrict__ kr_next,
-                                         const LaneTw<float, 8>& tw,
-                                         const C2<float>* __restrict__ rw32, const C2<double>* __restrict__ rw_own,
-                                         const C2<double>* __restrict__ rw_other, int wave, int lane, int split_bits,
+                                         const LaneTw<float, 8>& tw, const C2<float>* __restrict__ rw32,
+                                         const C2<double>* __restrict__ rws, int wave, int lane, int split_bits,
                                          double* raw, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
+    C2<float>* sbuf = (C2<float>*)sx_own;
+    const float hc = wave ? -0.70710678118654752440f : 0.70710678118654752440f;
     C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
@@ -1400,12 +1424,11 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
             const int row = q * L + lvl;
             C2<float> v[R1];
 #pragma unroll
-            for (int aa = 0; aa < R1; ++aa) {
-                const C2<float> p0{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl)};
-                const C2<float> p1{(float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl)};
-                v[aa] = w2_prep(p0, p1, rw32[aa], wave, aa == 0);
-            }
-            fft_fwd<R1>(v, tw, sx_own, lane);
+            for (int aa = 0; aa < R1; ++aa)
+                v[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
+                                  (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl), hc,
+                                  rw32[aa], aa == 0);
+            fft_fwd<R1>(v, tw, sbuf, lane);
             const float4* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
@@ -1427,12 +1450,14 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
         }
     }
     asm volatile("" ::"v"(sink));
-    fft_inv<R1>(h0, tw, sx_own, lane);
-    fft_inv<R1>(l0, tw, sx_own, lane);
-    fft_inv<R1>(h1, tw, sx_own, lane);
-    fft_inv<R1>(l1, tw, sx_own, lane);
+    fft_inv<R1>(h0, tw, sbuf, lane);
+    fft_inv<R1>(l0, tw, sbuf, lane);
+    fft_inv<R1>(h1, tw, sbuf, lane);
+    fft_inv<R1>(l1, tw, sbuf, lane);
     }
     const double hs = ldexp(1.0, split_bits);
+    C2<double>* xo = (C2<double>*)sx_own;
+    const C2<double>* xp = (const C2<double>*)sx_other;
     // one output polynomial at a time: exchange hi, combine and round it, then lo
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -1441,18 +1466,19 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, C2<float>* sx_own, cons
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
             const C2<float>* X = part ? (q ? l1 : l0) : (q ? h1 : h0);
+            C2<double> own[R1];
             if (work) {
 #pragma unroll
-                for (int aa = 0; aa < R1; ++aa) sx_own[aa * 64 + lane] = X[aa];
+                for (int aa = 0; aa < R1; ++aa) {
+                    own[aa] = cmulc(C2<double>{(double)X[aa].x, (double)X[aa].y}, rws[aa]);
+                    xo[aa * 64 + lane] = own[aa];
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
                 if (!work) break;
-                const C2<float> y = sx_other[aa * 64 + lane];
-                const C2<double> own = cmulc(C2<double>{(double)X[aa].x, (double)X[aa].y}, rw_own[aa]);
-                const C2<double> oth = cmulc(C2<double>{(double)y.x, (double)y.y}, rw_other[aa]);
-                const C2<double> z = w2_pair(own, oth, wave);
+                const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], wave);
                 if (!part) {
                     rh[2 * aa] = round_mod32(z.x);
                     rh[2 * aa + 1] = round_mod32(z.y);
@@ -1571,7 +1597,9 @@ k_boot2w(BootKArgs A) {
     constexpr int XB = xbuf_elems<W2_H>();
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
     __shared__ int32_t s_acc_all[GB][2 * N];
-    __shared__ double s_xd[GB][2][P ? XB : 2 * XB];   // per wavefront: C2<float>[XB] or C2<double>[XB]
+    // per wavefront C2<double>[XB]: FP64 transform exchanges; split-FP32 transform exchanges
+    // (C2<float>[XB]) and then its untwisted FP64 outputs (C2<double>[512])
+    __shared__ double s_xd[GB][2][2 * XB];
     __shared__ int16_t s_bara_all[GB][kMaxLweN];
     __shared__ C2<double> s_t2[P ? 1 : 8 * T2S];
     const int lane = threadIdx.x & 63;
@@ -1596,21 +1624,22 @@ k_boot2w(BootKArgs A) {
         LaneTw<float, 8> tw;
         load_lane_tw(tw, (const C2<float>*)A.t1 + wave * 64 * 8, (const C2<float>*)A.t2, lane);
         const C2<float>* rw32 = (const C2<float>*)A.t2w_rw32 + wave * 8;
-        C2<float>* sx_own = (C2<float>*)s_xd[b][wave];
-        const C2<float>* sx_other = (const C2<float>*)s_xd[b][1 - wave];
+        const C2<double>* rws = (const C2<double>*)A.untw64 + wave * 8;
+        void* sx_own = s_xd[b][wave];
+        const void* sx_other = s_xd[b][1 - wave];
         const float4* key = (const float4*)A.bk;
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw32, rw_own, rw_other, wave,
-                                   lane, A.split_bits, raw, active);
+            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw32, rws, wave, lane,
+                                   A.split_bits, raw, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
                 // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); a is the same in both
                 // wavefronts of a bootstrap, and an idle bootstrap still meets every barrier
                 const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
                 step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
-                                      key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw32, rw_own, rw_other,
-                                      wave, lane, A.split_bits, nullptr, a != 0);
+                                      key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw32, rws, wave, lane,
+                                      A.split_bits, nullptr, a != 0);
             }
         }
     } else {
@@ -1689,12 +1718,14 @@ __global__ void __launch_bounds__(128) k_keyprep2w(const int32_t* __restrict__ b
 }
 
 // Transform test kernel at N = 2048 (two wavefronts): forward int32[N] -> F_k natural order, or
-// inverse F natural -> polynomial (FP64 values).
+// inverse F natural -> polynomial (FP64 values), with the bootstrap's prep and output pairing of
+// each path (P = 1: split, whose wavefronts untwist their own outputs with rws64 and exchange them).
 template <int P>
 __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, const void* __restrict__ in, int inverse,
                                                     const void* __restrict__ t1w, const void* __restrict__ t2,
                                                     const C2<float>* __restrict__ rw32,
-                                                    const C2<double>* __restrict__ rw64) {
+                                                    const C2<double>* __restrict__ rw64,
+                                                    const C2<double>* __restrict__ rws64) {
     using T = typename std::conditional<P == 1, float, double>::type;
     constexpr int N = W2_N, R1 = 8;
     __shared__ C2<double> s_buf[2][xbuf_elems<W2_H>()];
@@ -1710,10 +1741,13 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
             const int j = lane + 64 * aa;
-            C2<T> rwa;
-            if constexpr (P == 1) rwa = rw32[wave * 8 + aa];
-            else rwa = rw64[wave * 8 + aa];
-            v[aa] = w2_prep(C2<T>{(T)p[j], (T)p[j + 1024]}, C2<T>{(T)p[j + 512], (T)p[j + 1536]}, rwa, wave, aa == 0);
+            if constexpr (P == 1)
+                v[aa] = w2_prep32((float)p[j], (float)p[j + 1024], (float)p[j + 512], (float)p[j + 1536],
+                                  wave ? -0.70710678118654752440f : 0.70710678118654752440f, rw32[wave * 8 + aa],
+                                  aa == 0);
+            else
+                v[aa] = w2_prep(C2<T>{(T)p[j], (T)p[j + 1024]}, C2<T>{(T)p[j + 512], (T)p[j + 1536]},
+                                rw64[wave * 8 + aa], wave, aa == 0);
         }
         fft_fwd<R1>(v, tw, s_buf[wave], lane);
 #pragma unroll
@@ -1732,17 +1766,33 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
         fft_inv<R1>(v, tw, s_buf[wave], lane);
         __syncthreads();
         C2<double>* xo = (C2<double>*)s_buf[wave];
-#pragma unroll
-        for (int aa = 0; aa < R1; ++aa) xo[aa * 64 + lane] = C2<double>{(double)v[aa].x, (double)v[aa].y};
-        __syncthreads();
         const C2<double>* xp = (const C2<double>*)s_buf[1 - wave];
+        if constexpr (P == 1) {
+            C2<double> own[R1];
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) {
-            const C2<double> own = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, rw64[wave * 8 + aa]);
-            const C2<double> oth = cmulc(xp[aa * 64 + lane], rw64[(1 - wave) * 8 + aa]);
-            const C2<double> z = w2_pair(own, oth, wave);
-            o[w2_coef(wave, lane, aa, 0)] = z.x;
-            o[w2_coef(wave, lane, aa, 1)] = z.y;
+            for (int aa = 0; aa < R1; ++aa) {
+                own[aa] = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, rws64[wave * 8 + aa]);
+                xo[aa * 64 + lane] = own[aa];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], wave);
+                o[w2_coef(wave, lane, aa, 0)] = z.x;
+                o[w2_coef(wave, lane, aa, 1)] = z.y;
+            }
+        } else {
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) xo[aa * 64 + lane] = C2<double>{(double)v[aa].x, (double)v[aa].y};
+            __syncthreads();
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const C2<double> own = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, rw64[wave * 8 + aa]);
+                const C2<double> oth = cmulc(xp[aa * 64 + lane], rw64[(1 - wave) * 8 + aa]);
+                const C2<double> z = w2_pair(own, oth, wave);
+                o[w2_coef(wave, lane, aa, 0)] = z.x;
+                o[w2_coef(wave, lane, aa, 1)] = z.y;
+            }
         }
     }
 }
@@ -1826,7 +1876,7 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
     if (key.p.N == 1024) return launch_boot_t<1024, 3, 7>(key, tw, prec, Lc, st);
     if (key.p.N == 2048) {
         // two wavefronts per bootstrap (k_boot2w); every mode is supported
-        BootKArgs A;
+        BootKArgs A{};
         A.L = Lc;
         A.n = key.p.n;
         A.split_bits = key.split_bits;
@@ -1842,6 +1892,7 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
             A.bk = key.bk32;
             A.t1 = tw.t1w32;
             A.t2 = tw.t2_32;
+            A.untw64 = tw.rws64;
             constexpr int GB = 4;   // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD
             hipLaunchKernelGGL((k_boot2w<3, 8, 1, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         }
@@ -1926,7 +1977,7 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
         else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test2w<0>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w64, tw.t2_64,
-                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64);
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64);
         else return hipErrorInvalidValue;
     } else {
         if (N == 1024)
@@ -1935,7 +1986,7 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<float>*)tw.twist32, (const C2<double>*)tw.untw64);
         else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test2w<1>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w32, tw.t2_32,
-                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64);
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();
