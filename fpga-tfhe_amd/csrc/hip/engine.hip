@@ -10,6 +10,8 @@
 //   k_boot64lat / k_boot32lat  latency forms for small launches: one bootstrap spread over a
 //                        workgroup of 2l wavefronts, one per digit polynomial.
 //   k_boot2w             N = 2048: each transform split over the two wavefronts of a bootstrap.
+//   k_boot2wlat          N = 2048 latency form: one bootstrap per workgroup, a wavefront pair
+//                        (the two spectrum halves) per digit level.
 //   k_keyswitch          bit-exact LWE keyswitch, a tile of ciphertexts per workgroup so each
 //                        keyswitch-key row is read once per tile (coalesced over coefficients);
 //                        k_keyswitch_sliced spreads a few ciphertexts' input indices over
@@ -1667,6 +1669,201 @@ k_boot2w(BootKArgs A) {
     if (active && wave == 0) boot_epilogue<N>(A, g, lane, s_acc);
 }
 
+// ---------------------------------------------------------------------------------------------
+// N = 2048 latency form: one bootstrap per workgroup.  Wavefront w < 2l is (level lvl = w / 2,
+// parity v = w % 2): for q = 0, 1 it rotates polynomial q, transforms its level-lvl digits into the
+// parity-v half of the spectrum (the two-wavefront split above) and multiplies them by the
+// parity-v half of key row (q, lvl), accumulating one product per output (FP64: the two output
+// polynomials; split: each polynomial times the hi and the lo key half).  After a barrier,
+// wavefront w < 2K takes parity v = w % 2 of output k = w / 2: it sums the l partials in a fixed
+// order, inverts them, untwists its own half and leaves it in LDS (over its own partials); after a
+// second barrier it pairs it with the partner half and adds the rounded coefficients it keeps into
+// the accumulator (split: LDS integer atomics, the hi half shifted by the split, exact mod 2^32 in
+// either order; FP64: each coefficient has one writer).  A step's critical path is two forward
+// half-transforms and one inverse half-transform instead of ten.  FP64 outputs equal k_boot2w's
+// bit for bit: both are the exact integers the final rounding recovers.  For launches of at most
+// one bootstrap per CU (modes 0, 1 and 3).
+// ---------------------------------------------------------------------------------------------
+template <int L, int BG, int P>
+__global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A) {
+    constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32, KPL = 2 * L;
+    constexpr int K = P ? 4 : 2;          // outputs per parity
+    constexpr int NW = P ? 8 : 2 * L;      // wavefronts: 2l forward, 2K inverse
+    static_assert(2 * K <= NW && 2 * L <= NW, "every forward and inverse job has a wavefront");
+    constexpr int XB = xbuf_elems<W2_H>();
+    constexpr int T2S = 9;
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    using T = typename std::conditional<P == 1, float, double>::type;
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ int16_t s_bara[kMaxLweN];
+    __shared__ double s_x[NW][XB];              // transform exchanges: C2<float>[XB] or double[XB] (half width)
+    __shared__ C2<T> s_part[2][K][L][W2_H];     // [parity][output][level][frequency]; then the output exchange
+    __shared__ C2<double> s_t2[P ? 1 : 8 * T2S];
+    static_assert(sizeof(s_part[0][0]) >= W2_H * sizeof(C2<double>), "an output half fits over its partials");
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int v = w & 1;
+    const int g = blockIdx.x;
+    if (w == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if constexpr (P == 0) {
+        if (threadIdx.x < 64) s_t2[(threadIdx.x >> 3) * T2S + (threadIdx.x & 7)] = ((const C2<double>*)A.t2)[threadIdx.x];
+    }
+    __syncthreads();
+    constexpr size_t ROWSET = (size_t)KPL * 2 * M;
+    using TW = typename std::conditional<P == 1, LaneTw<float, 8>, LaneTwS<double, 8>>::type;
+    TW tw;
+    if constexpr (P == 1) {
+        load_lane_tw(tw, (const C2<float>*)A.t1 + v * 64 * 8, (const C2<float>*)A.t2, lane);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) tw.t1[r] = ((const C2<double>*)A.t1)[(v * 64 + lane) * 8 + r];
+        tw.t2 = s_t2 + (lane & 7) * T2S;
+    }
+    const C2<double>* rw64 = (const C2<double>*)A.t2w_rw64 + v * 8;     // FP64: forward and untwist
+    const C2<float>* rw32 = (const C2<float>*)A.t2w_rw32 + v * 8;       // split: forward
+    const C2<double>* rws = (const C2<double>*)A.untw64 + v * 8;        // split: untwist, 1/1024 folded in
+    const float hc = v ? -0.70710678118654752440f : 0.70710678118654752440f;
+    const int n = A.n;
+    float sink = 0.f, warm[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < n; ++i) {
+        const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+        if (a == 0) continue;   // tfhe_blindRotate_FFT.cpp:114; uniform over the workgroup
+        if (w < 2 * L) {
+            const int lvl = w >> 1;
+            C2<T> acc[K][R1];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int r = 0; r < R1; ++r) acc[k][r] = C2<T>{(T)0, (T)0};
+            int t2o = 0;
+            asm volatile("" : "+v"(t2o));
+            TW twl = tw;
+            if constexpr (P == 0) twl.t2 = tw.t2 + t2o;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                uint32_t u[NC];
+#pragma unroll
+                for (int r = 0; r < NC; ++r)
+                    u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
+                C2<T> x[R1];
+                if constexpr (P == 1) {
+#pragma unroll
+                    for (int aa = 0; aa < R1; ++aa)
+                        x[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
+                                          (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl),
+                                          hc, rw32[aa], aa == 0);
+                    fft_fwd<R1>(x, twl, s_x[w], lane);
+                    const float4* kp = (const float4*)A.bk + (size_t)i * ROWSET + (size_t)((q * L + lvl) * 2) * M +
+                                       v * W2_H + lane;
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) {
+                        const float4 k0 = kp[r * 64];
+                        cmac(acc[0][r], x[r], C2<float>{k0.x, k0.y});
+                        cmac(acc[1][r], x[r], C2<float>{k0.z, k0.w});
+                    }
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) {
+                        const float4 k1 = kp[M + r * 64];
+                        cmac(acc[2][r], x[r], C2<float>{k1.x, k1.y});
+                        cmac(acc[3][r], x[r], C2<float>{k1.z, k1.w});
+                    }
+                } else {
+#pragma unroll
+                    for (int aa = 0; aa < R1; ++aa) {
+                        const C2<double> p0{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + 16], lvl)};
+                        const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
+                        x[aa] = w2_prep(p0, p1, rw64[aa], v, aa == 0);
+                    }
+                    fft_fwd<R1, double, true, false>(x, twl, s_x[w], lane);
+                    const C2<double>* kp = (const C2<double>*)A.bk + (size_t)i * ROWSET +
+                                           (size_t)((q * L + lvl) * 2) * M + v * W2_H + lane;
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(acc[0][r], x[r], kp[r * 64]);
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(acc[1][r], x[r], kp[M + r * 64]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int r = 0; r < R1; ++r) s_part[v][k][lvl][r * 64 + lane] = acc[k][r];
+            // a single bootstrap shares its key rows with nobody, so every row would come from
+            // memory: touch every 128-byte line of this wavefront's two rows of the next step now,
+            // a step before they are read.  The loaded words are only folded into `sink` at the
+            // next step, so nothing waits for them here.
+            if (i + 1 < n) {
+                const float4* nk = (const float4*)A.bk + (size_t)(i + 1) * ROWSET;
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const float* rp = (const float*)(nk + (size_t)((q * L + lvl) * 2) * M + v * W2_H);
+                    sink += warm[2 * q] + warm[2 * q + 1];
+                    warm[2 * q] = rp[lane * 32];
+                    warm[2 * q + 1] = rp[W2_M * 4 + lane * 32];
+                }
+            }
+        }
+        __syncthreads();
+        const int k = w >> 1;
+        C2<double> own[R1];
+        if (w < 2 * K) {
+            int t2o = 0;
+            asm volatile("" : "+v"(t2o));
+            TW twl = tw;
+            if constexpr (P == 0) twl.t2 = tw.t2 + t2o;
+            C2<T> o[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                o[r] = s_part[v][k][0][r * 64 + lane];
+#pragma unroll
+                for (int t = 1; t < L; ++t) o[r] = o[r] + s_part[v][k][t][r * 64 + lane];
+            }
+            C2<double>* xo = (C2<double>*)&s_part[v][k][0][0];
+            if constexpr (P == 1) {
+                fft_inv<R1>(o, twl, s_x[w], lane);
+#pragma unroll
+                for (int aa = 0; aa < R1; ++aa) {
+                    own[aa] = cmulc(C2<double>{(double)o[aa].x, (double)o[aa].y}, rws[aa]);
+                    xo[aa * 64 + lane] = own[aa];
+                }
+            } else {
+                fft_inv<R1, double, true, false>(o, twl, s_x[w], lane);
+#pragma unroll
+                for (int aa = 0; aa < R1; ++aa) {
+                    own[aa] = aa ? cmulc(o[aa], rw64[aa]) : o[aa];
+                    xo[aa * 64 + lane] = own[aa];
+                }
+            }
+        }
+        __syncthreads();
+        if (w < 2 * K) {
+            const C2<double>* xp = (const C2<double>*)&s_part[1 - v][k][0][0];
+            if constexpr (P == 1) {
+                const int j = k >> 1, sh = (k & 1) ? 0 : A.split_bits;   // k = 2j + (0 hi, 1 lo)
+                int32_t* accp = s_acc + j * N;
+#pragma unroll
+                for (int aa = 0; aa < R1; ++aa) {
+                    const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], v);
+                    atomicAdd(accp + w2_coef(v, lane, aa, 0), (int32_t)(round_mod32(z.x) << sh));
+                    atomicAdd(accp + w2_coef(v, lane, aa, 1), (int32_t)(round_mod32(z.y) << sh));
+                }
+            } else {
+                int32_t* accp = s_acc + k * N;
+#pragma unroll
+                for (int aa = 0; aa < R1; ++aa) {
+                    const C2<double> y = xp[aa * 64 + lane];
+                    const C2<double> z = w2_pair(own[aa], y, v);
+                    const int j0 = w2_coef(v, lane, aa, 0), j1 = w2_coef(v, lane, aa, 1);
+                    accp[j0] = (int32_t)((uint32_t)accp[j0] + round_mod32(z.x));
+                    accp[j1] = (int32_t)((uint32_t)accp[j1] + round_mod32(z.y));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    asm volatile("" ::"v"(sink + warm[0] + warm[1] + warm[2] + warm[3]));
+    if (w == 0) boot_epilogue<N>(A, g, lane, s_acc);
+}
+
 // Key preparation at N = 2048: one two-wavefront workgroup per key polynomial, the same prep and
 // 512-point transform as the bootstrap (FP64), stored as [parity][register][lane].
 __global__ void __launch_bounds__(128) k_keyprep2w(const int32_t* __restrict__ bk, C2<double>* __restrict__ out64,
@@ -1882,10 +2079,20 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
         A.split_bits = key.split_bits;
         A.t2w_rw64 = tw.rw64;
         A.t2w_rw32 = tw.rw32;
+        // launches of at most one bootstrap per CU take the latency form (FP64: unless a launch form
+        // other than it is forced, as at N = 1024)
+        const bool lat = Lc.mode != 2 && A.n <= kMaxLweN &&
+                         (prec == PREC_FP64 ? (Lc.form == 3 || (Lc.form == 0 && g_boot64_shared == 2 &&
+                                                                Lc.count <= boot64_latency_max()))
+                                            : Lc.count <= boot64_latency_max());
         if (prec == PREC_FP64) {
             A.bk = key.bk64;
             A.t1 = tw.t1w64;
             A.t2 = tw.t2_64;
+            if (lat) {
+                hipLaunchKernelGGL((k_boot2wlat<3, 8, 0>), dim3(Lc.count), dim3(64 * 6), 0, st, A);
+                return hipGetLastError();
+            }
             constexpr int GB = 4;   // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD
             hipLaunchKernelGGL((k_boot2w<3, 8, 0, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         } else {
@@ -1893,6 +2100,10 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
             A.t1 = tw.t1w32;
             A.t2 = tw.t2_32;
             A.untw64 = tw.rws64;
+            if (lat) {
+                hipLaunchKernelGGL((k_boot2wlat<3, 8, 1>), dim3(Lc.count), dim3(64 * 8), 0, st, A);
+                return hipGetLastError();
+            }
             constexpr int GB = 4;   // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD
             hipLaunchKernelGGL((k_boot2w<3, 8, 1, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
         }

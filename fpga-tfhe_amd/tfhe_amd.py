@@ -258,7 +258,8 @@ def bk_from_lagrange(flat_params, lagrange):
 
 def set_fp64_shared_rows(on):
     """FP64 launch form: True (key-sharing workgroups), False (one bootstrap per workgroup) or
-    "auto" (the default: by launch size)."""
+    "auto" (the default: by launch size).  At N=2048, True and False both select the batched
+    two-wavefront kernel and "auto" adds the latency form for small launches."""
     _ck(lib().tfhe_gpu_set_fp64_shared_rows(2 if on == "auto" else (1 if on else 0)), "set_fp64_shared_rows")
 
 

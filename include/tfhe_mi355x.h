@@ -348,7 +348,10 @@ int tfhe_gpu_set_default_precision(int32_t prec);
  * 0: one bootstrap per workgroup with a private LDS key-row buffer;
  * 2 (default): by launch size: up to 1 bootstrap per CU, a latency form that spreads each
  *    bootstrap over six wavefronts; up to 4 per CU (what form 0 runs at once), form 0; above
- *    that, form 1.  Results are identical in every form. */
+ *    that, form 1.  Results are identical in every form.
+ * These forms are those of N = 1024.  At N = 2048 the FP64 path runs the latency form for launches
+ * of up to 1 bootstrap per CU under 2, and otherwise (and under 0 or 1) the batched kernel that
+ * spreads each bootstrap over two wavefronts; results are identical there too. */
 int tfhe_gpu_set_fp64_shared_rows(int32_t on);
 
 /* flat views of host key sets made by new_random_gate_bootstrapping_secret_keyset (any output may

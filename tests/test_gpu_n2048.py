@@ -125,3 +125,67 @@ def test_gate_n2048_fp64_bit_exact_vs_oracle(env2048, gate):
     out = dk.gate(gate, ca, cb, cc if gate == "MUX" else None, prec=0)
     for i in range(2):
         assert np.array_equal(out[i], O.gate(keys, gate, ca[i], cb[i], cc[i])), i
+
+
+@pytest.mark.parametrize("cnt", [2, 300])
+def test_blind_rotate_n2048_fp64_forms(env2048, cnt):
+    """Launches of at most one bootstrap per CU run the latency form (one workgroup per bootstrap,
+    a wavefront pair per level), larger ones the batched two-wavefront kernel.  Accumulators 0, 1
+    and the last are bit-exact with the oracle; on the small launch the latency form and the
+    batched kernel (forced through the FP64 form switch) agree bit for bit."""
+    _, dk, keys = env2048
+    rng = np.random.default_rng(2055 + cnt)
+    acc = np.stack([O.test_vector(P, int(b), T().MU) for b in rng.integers(0, 2 * N, cnt)])
+    bara = rng.integers(0, 2 * N, (cnt, P.n)).astype(np.int32)
+    bara[0, ::5] = 0
+    got = dk.blind_rotate(acc, bara, prec=0)
+    for g in sorted({0, 1, cnt - 1}):
+        assert np.array_equal(got[g], O.blind_rotate(keys, acc[g], bara[g])), g
+    if cnt <= 256:
+        T().set_fp64_shared_rows(True)
+        try:
+            batched = dk.blind_rotate(acc, bara, prec=0)
+        finally:
+            T().set_fp64_shared_rows("auto")
+        assert np.array_equal(batched, got)
+
+
+@pytest.mark.parametrize("cnt", [2, 300])
+def test_blind_rotate_n2048_split32_phase_error(env2048, cnt):
+    """Split FP32 at N=2048, latency form (2 accumulators) and batched kernel (300): the TLWE phase
+    of accumulators 0, 1 and the last against the noiseless rotation, beside the FP64 path's.  At
+    this set the key noise is small (sigma 2^-30: FP64 rms about 2^20 units) and the split path's
+    own error dominates, about 2^24.5 units rms after the 750 steps."""
+    _, dk, keys = env2048
+    rng = np.random.default_rng(2057 + cnt)
+    acc = np.stack([O.test_vector(P, int(b), T().MU) for b in rng.integers(0, 2 * N, cnt)])
+    bara = rng.integers(0, 2 * N, (cnt, P.n)).astype(np.int32)
+    res = {p: dk.blind_rotate(acc, bara, prec=p) for p in (0, 1)}
+    rms, worst, mean = {}, {}, {}
+    for p in (0, 1):
+        errs = []
+        for g in sorted({0, 1, cnt - 1}):
+            rot = int((bara[g].astype(np.int64) * keys.lwe_key).sum() % (2 * N))
+            ideal = O.poly_mul_xai(rot, acc[g, 1])
+            ph = O.tlwe_phase(P, res[p][g], keys.tlwe_key)
+            errs.append((ph.astype(np.int64) - ideal.astype(np.int64) + 2**31) % 2**32 - 2**31)
+        d = np.concatenate(errs).astype(np.float64)
+        e = np.abs(d)
+        rms[p], worst[p], mean[p] = np.sqrt((e ** 2).mean()), e.max(), d.mean()
+    print(f"N=2048 blind rotation phase noise (2^-32 units), count {cnt}: FP64 rms {rms[0]:.3e} max {worst[0]:.3e}; "
+          f"split rms {rms[1]:.3e} max {worst[1]:.3e} mean {mean[1]:.3e}")
+    assert rms[0] < 2**22
+    # stated bounds: split rms within 2^25 and no coefficient beyond 2^27, half of the 1/16 gate
+    # decision margin (2^28 units)
+    assert rms[1] < 2**25
+    assert worst[1] < 2**27
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gate_n2048_batched_nand(env2048, prec):
+    """300 NANDs (more than one bootstrap per CU: the batched kernel) decrypt correctly."""
+    ks, dk, _ = env2048
+    rng = np.random.default_rng(2059)
+    bits = rng.integers(0, 2, (2, 300)).astype(np.int32)
+    out = dk.gate("NAND", ks.encrypt(bits[0]), ks.encrypt(bits[1]), prec=prec)
+    assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
