@@ -1752,18 +1752,19 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                         x[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
                                           (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl),
                                           hc, rw32[aa], aa == 0);
-                    fft_fwd<R1>(x, twl, s_x[w], lane);
+                    // the row's key values are requested before the transform, whose length hides
+                    // their latency
                     const float4* kp = (const float4*)A.bk + (size_t)i * ROWSET + (size_t)((q * L + lvl) * 2) * M +
                                        v * W2_H + lane;
+                    float4 k0[R1];
 #pragma unroll
-                    for (int r = 0; r < R1; ++r) {
-                        const float4 k0 = kp[r * 64];
-                        cmac(acc[0][r], x[r], C2<float>{k0.x, k0.y});
-                        cmac(acc[1][r], x[r], C2<float>{k0.z, k0.w});
-                    }
+                    for (int r = 0; r < R1; ++r) k0[r] = kp[r * 64];
+                    fft_fwd<R1>(x, twl, s_x[w], lane);
 #pragma unroll
                     for (int r = 0; r < R1; ++r) {
                         const float4 k1 = kp[M + r * 64];
+                        cmac(acc[0][r], x[r], C2<float>{k0[r].x, k0[r].y});
+                        cmac(acc[1][r], x[r], C2<float>{k0[r].z, k0[r].w});
                         cmac(acc[2][r], x[r], C2<float>{k1.x, k1.y});
                         cmac(acc[3][r], x[r], C2<float>{k1.z, k1.w});
                     }
@@ -1774,11 +1775,14 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                         const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                         x[aa] = w2_prep(p0, p1, rw64[aa], v, aa == 0);
                     }
-                    fft_fwd<R1, double, true, false>(x, twl, s_x[w], lane);
                     const C2<double>* kp = (const C2<double>*)A.bk + (size_t)i * ROWSET +
                                            (size_t)((q * L + lvl) * 2) * M + v * W2_H + lane;
+                    C2<double> k0[R1];
 #pragma unroll
-                    for (int r = 0; r < R1; ++r) cmac(acc[0][r], x[r], kp[r * 64]);
+                    for (int r = 0; r < R1; ++r) k0[r] = kp[r * 64];
+                    fft_fwd<R1, double, true, false>(x, twl, s_x[w], lane);
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(acc[0][r], x[r], k0[r]);
 #pragma unroll
                     for (int r = 0; r < R1; ++r) cmac(acc[1][r], x[r], kp[M + r * 64]);
                 }
