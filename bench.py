@@ -213,6 +213,16 @@ def n2048_report(T, gates, steps=2):
             res[name] = {"ms_per_step": round(sec * 1e3, 2), "gates_per_s": round(2 * gates / sec, 1),
                          "bootstraps_per_s": round(3 * gates / sec, 1), "failures": fails,
                          "gates_checked": 2 * gates}
+            # one NAND per call from host arrays (the upstream single-gate API's shape): the
+            # small-launch forms and the copies, end to end
+            a1, b1 = ks.encrypt(bits[0][:1]), ks.encrypt(bits[1][:1])
+            dk.gate("NAND", a1, b1, prec=prec)
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                o1 = dk.gate("NAND", a1, b1, prec=prec)
+            res[name]["single_gate_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+            res[name]["single_gate_correct"] = bool(ks.decrypt(o1)[0] == 1 - (bits[0][0] & bits[1][0]))
             log(f"[bench] N=2048 {name}: {res[name]}")
         acc = rng.integers(-2**31, 2**31, (32, 2, 2048), dtype=np.int64).astype(np.int32)
         rows = rng.integers(0, nn, 32).astype(np.int32)
