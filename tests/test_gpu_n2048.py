@@ -189,3 +189,105 @@ def test_gate_n2048_batched_nand(env2048, prec):
     bits = rng.integers(0, 2, (2, 300)).astype(np.int32)
     out = dk.gate("NAND", ks.encrypt(bits[0]), ks.encrypt(bits[1]), prec=prec)
     assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_upstream_api_n2048(env2048, prec):
+    """bootsSymEncrypt / bootsCONSTANT / bootsMUX / bootsNAND / bootsXOR on an N=2048 key set, one
+    gate per call (the latency form and the sliced keyswitch), outputs written over inputs."""
+    import ctypes
+    ks, _, _ = env2048
+    L = T().lib()
+    L.tfhe_gpu_set_default_precision(prec)
+    arr = L.new_gate_bootstrapping_ciphertext_array(4, ks.params_ptr)
+    try:
+        L.bootsSymEncrypt(ctypes.byref(arr[0]), 1, ks.ptr)
+        L.bootsSymEncrypt(ctypes.byref(arr[1]), 0, ks.ptr)
+        L.bootsCONSTANT(ctypes.byref(arr[2]), 1, ks.cloud_ptr)
+        print(f"[n2048] upstream gates prec={prec}", flush=True)
+        # MUX(sel=1, b=0, c=1) = 0
+        L.bootsMUX(ctypes.byref(arr[3]), ctypes.byref(arr[0]), ctypes.byref(arr[1]), ctypes.byref(arr[2]),
+                   ks.cloud_ptr)
+        assert L.bootsSymDecrypt(ctypes.byref(arr[3]), ks.ptr) == 0
+        for name in ("bootsNAND", "bootsXOR"):
+            getattr(L, name).argtypes = [ctypes.c_void_p] * 4
+        L.bootsXOR(ctypes.addressof(arr[1]), ctypes.addressof(arr[1]), ctypes.addressof(arr[2]), ks.cloud_ptr)
+        assert L.bootsSymDecrypt(ctypes.byref(arr[1]), ks.ptr) == 1          # XOR(0, 1)
+        L.bootsNAND(ctypes.addressof(arr[0]), ctypes.addressof(arr[0]), ctypes.addressof(arr[2]), ks.cloud_ptr)
+        assert L.bootsSymDecrypt(ctypes.byref(arr[0]), ks.ptr) == 0          # NAND(1, 1)
+    finally:
+        L.delete_gate_bootstrapping_ciphertext_array(4, arr)
+        L.tfhe_gpu_set_default_precision(0)
+
+
+def _adder_netlist(bits):
+    """Ripple-carry adder as a product circuit plus the recorded netlist: p_i = XOR(a_i, b_i),
+    s_i = XOR(p_i, c_i), c_{i+1} = MUX(p_i, c_i, a_i), c_0 = 0."""
+    c = T().Circuit(2 * bits)
+    ops = [("IN", i) for i in range(2 * bits)]
+
+    def add(op, a, b=-1, cc=-1):
+        w = c.add(op, a, b, cc)
+        assert w == len(ops)
+        ops.append((op, a, b, cc))
+        return w
+
+    p = [add("XOR", i, bits + i) for i in range(bits)]
+    carry = c.constant(0)
+    assert carry == len(ops)
+    ops.append(("CONST", 0))
+    outs = []
+    for i in range(bits):
+        s = add("XOR", p[i], carry)
+        c.output(s)
+        outs.append(s)
+        carry = add("MUX", p[i], carry, i)
+    c.output(carry)
+    outs.append(carry)
+    return c, ops, outs
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_circuit_n2048_adder_graph_replay(env2048, prec):
+    """A 4-bit adder at N=2048 over 3 instances, levels small enough for the latency form, captured
+    in a HIP graph (with the FP64 form autotuner) and replayed on fresh inputs in the bound buffer:
+    every replay equals an eager run and decrypts to the plain sums; FP64 outputs of one instance
+    are bit-exact with the oracle evaluating the netlist gate by gate."""
+    ks, dk, keys = env2048
+    rng = np.random.default_rng(2060 + prec)
+    c, ops, outs = _adder_netlist(4)
+    B, n1 = 3, P.n + 1
+    d_in = T().DeviceBuffer(8 * B * n1 * 4)
+    d_out = T().DeviceBuffer(5 * B * n1 * 4)
+    try:
+        for rep in range(2):
+            x, y = rng.integers(0, 16, B), rng.integers(0, 16, B)
+            bits = np.array([[(int(v) >> i) & 1 for v in vals] for vals in (x, y) for i in range(4)], np.int32)
+            ct = np.stack([ks.encrypt(bits[i]) for i in range(8)])
+            eager = c.run(dk, ct, prec=prec)
+            T().lib().tfhe_gpu_memcpy_h2d(d_in.ptr, ct.ctypes.data_as(T()._vp), ct.nbytes)
+            print(f"[n2048] adder graph run {rep} prec={prec}", flush=True)
+            c.run_dev(dk, d_in, d_out, B, prec=prec, graph=True, autotune=True)
+            T().sync()
+            got = d_out.to_array((5, B, n1))
+            assert np.array_equal(got, eager), rep
+            dec = np.stack([ks.decrypt(got[o]) for o in range(5)])
+            sums = [int(sum(int(dec[o][j]) << o for o in range(5))) for j in range(B)]
+            assert sums == [int(a + b) for a, b in zip(x, y)], rep
+        if prec == 0:
+            vals = []
+            for op in ops:
+                if op[0] == "IN":
+                    vals.append(ct[op[1], 0])
+                elif op[0] == "CONST":
+                    t = np.zeros(n1, np.int32)
+                    t[P.n] = T().MU if op[1] else -T().MU
+                    vals.append(t)
+                else:
+                    name, a, b, cc = op
+                    vals.append(O.gate(keys, name, vals[a], vals[b], vals[cc] if name == "MUX" else None))
+            assert np.array_equal(got[:, 0], np.stack([vals[w] for w in outs]))
+    finally:
+        d_in.free()
+        d_out.free()
+        c.free()
