@@ -1332,6 +1332,9 @@ bool params_supported(const DevParams& p) {
 // which needs one exchange of the inverse outputs through LDS.
 // ---------------------------------------------------------------------------------------------
 constexpr int W2_N = 2048, W2_M = 1024, W2_H = 512;
+// offset (in C2<float>) of the upper half of a split-path wavefront buffer: past the transform
+// exchange area, 4 KB of C2<float>[512] to the buffer's end
+constexpr int W2_XHALF = 576;
 
 template <typename T>
 __device__ __forceinline__ C2<T> w2_c() {
@@ -1391,8 +1394,8 @@ __device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, floa
 
 // Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.  Each wavefront untwists
 // its own inverse outputs in FP64 (rws: register constants with the 1/1024) and the untwisted values
-// are exchanged, so no wavefront untwists its partner's.  sx_own / sx_other: 8 KB per wavefront,
-// the transform exchange buffer (C2<float>) and then the output exchange (C2<double>[512]).
+// are exchanged, so no wavefront untwists its partner's.  sx_own / sx_other: 9 KB per wavefront,
+// the transform exchange buffer (C2<float>) and two halves for the output exchange.
 template <int L, int BG, bool ROT>
 __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const void* sx_other, int a,
                                          const float4* __restrict__ kr, const float4* __restrict__ kr_next,
@@ -1458,9 +1461,10 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     fft_inv<R1>(l1, tw, sbuf, lane);
     }
     const double hs = ldexp(1.0, split_bits);
-    C2<double>* xo = (C2<double>*)sx_own;
-    const C2<double>* xp = (const C2<double>*)sx_other;
-    // one output polynomial at a time: exchange hi, combine and round it, then lo
+    // one output polynomial at a time: exchange hi, combine and round it, then lo.  The
+    // exchanged values are rounded to FP32 and alternate between two 4 KB halves of the buffer, so
+    // a round needs one barrier: a half is written again two rounds later, after the partner has
+    // passed the barrier that follows its read.
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t rh[2 * R1];
@@ -1468,19 +1472,22 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
             const C2<float>* X = part ? (q ? l1 : l0) : (q ? h1 : h0);
+            C2<float>* xo = (C2<float>*)sx_own + part * W2_XHALF;
+            const C2<float>* xp = (const C2<float>*)sx_other + part * W2_XHALF;
             C2<double> own[R1];
             if (work) {
 #pragma unroll
                 for (int aa = 0; aa < R1; ++aa) {
                     own[aa] = cmulc(C2<double>{(double)X[aa].x, (double)X[aa].y}, rws[aa]);
-                    xo[aa * 64 + lane] = own[aa];
+                    xo[aa * 64 + lane] = C2<float>{(float)own[aa].x, (float)own[aa].y};
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
                 if (!work) break;
-                const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], wave);
+                const C2<float> o = xp[aa * 64 + lane];
+                const C2<double> z = w2_pair_own(own[aa], C2<double>{(double)o.x, (double)o.y}, wave);
                 if (!part) {
                     rh[2 * aa] = round_mod32(z.x);
                     rh[2 * aa + 1] = round_mod32(z.y);
@@ -1499,9 +1506,11 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                     }
                 }
             }
-            __syncthreads();
         }
     }
+    // the partner reads the upper half after the last barrier above, and every accumulator
+    // coefficient is read at the start of the next step
+    __syncthreads();
 }
 
 // FP64 step at N = 2048, one wavefront's share.
@@ -1600,7 +1609,7 @@ k_boot2w(BootKArgs A) {
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
     __shared__ int32_t s_acc_all[GB][2 * N];
     // per wavefront C2<double>[XB]: FP64 transform exchanges; split-FP32 transform exchanges
-    // (C2<float>[XB]) and then its untwisted FP64 outputs (C2<double>[512])
+    // (C2<float>[XB]) and its untwisted outputs, rounded to FP32, in two alternating halves
     __shared__ double s_xd[GB][2][2 * XB];
     __shared__ int16_t s_bara_all[GB][kMaxLweN];
     __shared__ C2<double> s_t2[P ? 1 : 8 * T2S];
@@ -1969,16 +1978,20 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
         C2<double>* xo = (C2<double>*)s_buf[wave];
         const C2<double>* xp = (const C2<double>*)s_buf[1 - wave];
         if constexpr (P == 1) {
+            // as the split bootstrap step: own half untwisted in FP64, exchanged rounded to FP32
             C2<double> own[R1];
+            C2<float>* xo32 = (C2<float>*)xo;
+            const C2<float>* xp32 = (const C2<float>*)xp;
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
                 own[aa] = cmulc(C2<double>{(double)v[aa].x, (double)v[aa].y}, rws64[wave * 8 + aa]);
-                xo[aa * 64 + lane] = own[aa];
+                xo32[aa * 64 + lane] = C2<float>{(float)own[aa].x, (float)own[aa].y};
             }
             __syncthreads();
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
-                const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], wave);
+                const C2<float> y = xp32[aa * 64 + lane];
+                const C2<double> z = w2_pair_own(own[aa], C2<double>{(double)y.x, (double)y.y}, wave);
                 o[w2_coef(wave, lane, aa, 0)] = z.x;
                 o[w2_coef(wave, lane, aa, 1)] = z.y;
             }
