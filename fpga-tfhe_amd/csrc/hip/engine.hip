@@ -1433,13 +1433,17 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                 v[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
                                   (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl), hc,
                                   rw32[aa], aa == 0);
-            fft_fwd<R1>(v, tw, sbuf, lane);
+            // the row's first key polynomial is requested before the transform, whose length hides
+            // its latency; the second after it (requesting both spills at 256 VGPRs)
             const float4* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
+            float4 k0[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
+            fft_fwd<R1>(v, tw, sbuf, lane);
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
-                const float4 k0 = k[r * 64];
-                cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
-                cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
+                cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
+                cmac(l0[r], v[r], C2<float>{k0[r].z, k0[r].w});
             }
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
@@ -1551,10 +1555,16 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
                 const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                 v[aa] = w2_prep(p0, p1, rw_own[aa], wave, aa == 0);
             }
-            fft_fwd<R1, double, false, false>(v, tw, sx_own, lane);
             const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
+            // the row's first EK key values are requested before the transform, whose length hides
+            // their latency (more spill at 256 VGPRs); the rest after it
+            constexpr int EK = 2;
+            C2<double> k0[EK];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], k[r * 64]);
+            for (int r = 0; r < EK; ++r) k0[r] = k[r * 64];
+            fft_fwd<R1, double, false, false>(v, tw, sx_own, lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) cmac(o0[r], v[r], r < EK ? k0[r < EK ? r : 0] : k[r * 64]);
 #pragma unroll
             for (int r = 0; r < R1; ++r) cmac(o1[r], v[r], k[M + r * 64]);
             __builtin_amdgcn_sched_barrier(0);
