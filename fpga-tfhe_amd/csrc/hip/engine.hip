@@ -1771,8 +1771,8 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                         x[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
                                           (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl),
                                           hc, rw32[aa], aa == 0);
-                    // the row's key values are requested before the transform, whose length hides
-                    // their latency
+                    // the row's first key polynomial is requested before the transform, whose length
+                    // hides its latency; the second after it (both would spill)
                     const float4* kp = (const float4*)A.bk + (size_t)i * ROWSET + (size_t)((q * L + lvl) * 2) * M +
                                        v * W2_H + lane;
                     float4 k0[R1];
@@ -1794,6 +1794,7 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                         const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                         x[aa] = w2_prep(p0, p1, rw64[aa], v, aa == 0);
                     }
+                    // as above: the first key polynomial before the transform, the second after it
                     const C2<double>* kp = (const C2<double>*)A.bk + (size_t)i * ROWSET +
                                            (size_t)((q * L + lvl) * 2) * M + v * W2_H + lane;
                     C2<double> k0[R1];
