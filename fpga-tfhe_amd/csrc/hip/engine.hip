@@ -1416,46 +1416,60 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     // an idle bootstrap (zero rotation, or past the batch end) skips the arithmetic but still takes
     // part in every workgroup barrier below
     if (work) {
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        uint32_t u[NC];
+    uint32_t u[NC];
+    auto rotate = [&](int q) {
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
             const int j = lane + 64 * r;
             u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
         }
+    };
+    auto prep = [&](C2<float>* x, int lvl) {
 #pragma unroll
-        for (int lvl = 0; lvl < L; ++lvl) {
-            const int row = q * L + lvl;
-            C2<float> v[R1];
+        for (int aa = 0; aa < R1; ++aa)
+            x[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
+                              (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl), hc,
+                              rw32[aa], aa == 0);
+    };
+    // rows t = (q, lvl) in order, software-pipelined: the row's first key polynomial is requested
+    // before its transform, the second after it, and the next row's digits are prepared (rotating
+    // the next polynomial first when q changes) while the second polynomial is in flight
+    C2<float> v[R1];
+    rotate(0);
+    prep(v, 0);
 #pragma unroll
-            for (int aa = 0; aa < R1; ++aa)
-                v[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
-                                  (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl), hc,
-                                  rw32[aa], aa == 0);
-            // the row's first key polynomial is requested before the transform, whose length hides
-            // its latency; the second after it (requesting both spills at 256 VGPRs)
-            const float4* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
-            float4 k0[R1];
+    for (int row = 0; row < 2 * L; ++row) {
+        const float4* k = kr + (size_t)(row * 2) * M + wave * W2_H + lane;
+        float4 k0[R1];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
-            fft_fwd<R1>(v, tw, sbuf, lane);
+        for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
+        fft_fwd<R1>(v, tw, sbuf, lane);
 #pragma unroll
-            for (int r = 0; r < R1; ++r) {
-                cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
-                cmac(l0[r], v[r], C2<float>{k0[r].z, k0[r].w});
-            }
+        for (int r = 0; r < R1; ++r) {
+            cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
+            cmac(l0[r], v[r], C2<float>{k0[r].z, k0[r].w});
+        }
+        float4 k1[R1];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) {
-                const float4 k1 = k[M + r * 64];
-                cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
-                cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
-            }
-            // issued after this row's key loads (vmcnt drains in issue order), so it overlaps the
-            // next transform and gates nothing but the next row's loads
-            __builtin_amdgcn_sched_barrier(0);
-            w2_warm_row(row + 1 < 2 * L ? kr + (size_t)((row + 1) * 2) * M + wave * W2_H : kr_next + wave * W2_H, lane,
-                        sink);
+        for (int r = 0; r < R1; ++r) k1[r] = k[M + r * 64];
+        C2<float> vn[R1];
+        if (row + 1 < 2 * L) {
+            if ((row + 1) % L == 0) rotate((row + 1) / L);
+            prep(vn, (row + 1) % L);
+        }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            cmac(h1[r], v[r], C2<float>{k1[r].x, k1[r].y});
+            cmac(l1[r], v[r], C2<float>{k1[r].z, k1[r].w});
+        }
+        // issued after this row's key loads (vmcnt drains in issue order), so it overlaps the
+        // next transform and gates nothing but the next row's loads
+        __builtin_amdgcn_sched_barrier(0);
+        w2_warm_row(row + 1 < 2 * L ? kr + (size_t)((row + 1) * 2) * M + wave * W2_H : kr_next + wave * W2_H, lane,
+                    sink);
+        if (row + 1 < 2 * L) {
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[r] = vn[r];
         }
     }
     asm volatile("" ::"v"(sink));
