@@ -1772,21 +1772,29 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
             asm volatile("" : "+v"(t2o));
             TW twl = tw;
             if constexpr (P == 0) twl.t2 = tw.t2 + t2o;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            if constexpr (P == 1) {
+                // the two rows (0, lvl) and (1, lvl), software-pipelined: each row's first key
+                // polynomial is requested before its transform, the second after it, and polynomial
+                // 1 is rotated and its digits prepared while row (0, lvl)'s second polynomial is in
+                // flight
                 uint32_t u[NC];
+                auto rotate = [&](int q) {
 #pragma unroll
-                for (int r = 0; r < NC; ++r)
-                    u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
-                C2<T> x[R1];
-                if constexpr (P == 1) {
+                    for (int r = 0; r < NC; ++r)
+                        u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
+                };
+                auto prep = [&](C2<float>* x) {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa)
                         x[aa] = w2_prep32((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl),
                                           (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl),
                                           hc, rw32[aa], aa == 0);
-                    // the row's first key polynomial is requested before the transform, whose length
-                    // hides its latency; the second after it (both would spill)
+                };
+                C2<float> x[R1];
+                rotate(0);
+                prep(x);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
                     const float4* kp = (const float4*)A.bk + (size_t)i * ROWSET + (size_t)((q * L + lvl) * 2) * M +
                                        v * W2_H + lane;
                     float4 k0[R1];
@@ -1795,20 +1803,44 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                     fft_fwd<R1>(x, twl, s_x[w], lane);
 #pragma unroll
                     for (int r = 0; r < R1; ++r) {
-                        const float4 k1 = kp[M + r * 64];
                         cmac(acc[0][r], x[r], C2<float>{k0[r].x, k0[r].y});
                         cmac(acc[1][r], x[r], C2<float>{k0[r].z, k0[r].w});
-                        cmac(acc[2][r], x[r], C2<float>{k1.x, k1.y});
-                        cmac(acc[3][r], x[r], C2<float>{k1.z, k1.w});
                     }
-                } else {
+                    float4 k1[R1];
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) k1[r] = kp[M + r * 64];
+                    C2<float> xn[R1];
+                    if (q == 0) {
+                        rotate(1);
+                        prep(xn);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) {
+                        cmac(acc[2][r], x[r], C2<float>{k1[r].x, k1[r].y});
+                        cmac(acc[3][r], x[r], C2<float>{k1[r].z, k1[r].w});
+                    }
+                    if (q == 0) {
+#pragma unroll
+                        for (int r = 0; r < R1; ++r) x[r] = xn[r];
+                    }
+                }
+            } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                uint32_t u[NC];
+#pragma unroll
+                for (int r = 0; r < NC; ++r)
+                    u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
+                C2<T> x[R1];
+                {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa) {
                         const C2<double> p0{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + 16], lvl)};
                         const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                         x[aa] = w2_prep(p0, p1, rw64[aa], v, aa == 0);
                     }
-                    // as above: the first key polynomial before the transform, the second after it
+                    // the first key polynomial is requested before the transform, the second after
+                    // it (FP64 has no room to pipeline the rows as the split path does)
                     const C2<double>* kp = (const C2<double>*)A.bk + (size_t)i * ROWSET +
                                            (size_t)((q * L + lvl) * 2) * M + v * W2_H + lane;
                     C2<double> k0[R1];
@@ -1820,6 +1852,7 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
 #pragma unroll
                     for (int r = 0; r < R1; ++r) cmac(acc[1][r], x[r], kp[M + r * 64]);
                 }
+            }
             }
 #pragma unroll
             for (int k = 0; k < K; ++k)
