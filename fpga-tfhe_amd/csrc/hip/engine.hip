@@ -1435,14 +1435,19 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     // before its transform, the second after it, and the next row's digits are prepared (rotating
     // the next polynomial first when q changes) while the second polynomial is in flight
     C2<float> v[R1];
+    float4 k0[R1];
+    // row 0's first key polynomial is requested before the rotation, which then hides it too
+#pragma unroll
+    for (int r = 0; r < R1; ++r) k0[r] = kr[wave * W2_H + lane + r * 64];
     rotate(0);
     prep(v, 0);
 #pragma unroll
     for (int row = 0; row < 2 * L; ++row) {
         const float4* k = kr + (size_t)(row * 2) * M + wave * W2_H + lane;
-        float4 k0[R1];
+        if (row > 0) {
 #pragma unroll
-        for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
+            for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
+        }
         fft_fwd<R1>(v, tw, sbuf, lane);
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
