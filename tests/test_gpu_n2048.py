@@ -182,13 +182,19 @@ def test_blind_rotate_n2048_split32_phase_error(env2048, cnt):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_gate_n2048_batched_nand(env2048, prec):
-    """300 NANDs (more than one bootstrap per CU: the batched kernel) decrypt correctly."""
-    ks, dk, _ = env2048
-    rng = np.random.default_rng(2059)
-    bits = rng.integers(0, 2, (2, 300)).astype(np.int32)
-    out = dk.gate("NAND", ks.encrypt(bits[0]), ks.encrypt(bits[1]), prec=prec)
+@pytest.mark.parametrize("cnt", [300, 259])
+def test_gate_n2048_batched_nand(env2048, prec, cnt):
+    """NAND batches above one bootstrap per CU run the batched kernel, four bootstraps per
+    workgroup; 259 leaves one idle slot in the last workgroup (it still meets every barrier).  All
+    outputs decrypt correctly; on FP64 the last gate is bit-exact with the oracle."""
+    ks, dk, keys = env2048
+    rng = np.random.default_rng(2059 + cnt)
+    bits = rng.integers(0, 2, (2, cnt)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    out = dk.gate("NAND", ca, cb, prec=prec)
     assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
+    if prec == 0:
+        assert np.array_equal(out[cnt - 1], O.gate(keys, "NAND", ca[cnt - 1], cb[cnt - 1], None))
 
 
 @pytest.mark.parametrize("prec", [0, 1])
