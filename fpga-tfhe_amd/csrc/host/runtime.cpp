@@ -424,7 +424,7 @@ extern "C" int tfhe_gpu_set_default_precision(int32_t prec) {
 // (k+1) l N terms) must stay below ~2^21 for FP32 to round it to the exact integer: with
 // Khi < 2^(31-s), Bg/2 = 64 and N = 1024 that holds for s = 24; N = 2048 with Bg/2 = 128 grows the
 // sum by ~2^2, so its hi half is made 2 bits narrower.
-static int split_bits_for(const DevParams& p) { return p.N <= 1024 ? 24 : 26; }
+static int split_bits_for(const DevParams& p) { return p.N <= 1024 ? 24 : 25; }
 
 extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk, const int32_t* ksk,
                                    int32_t prec_mask, tfhe_gpu_key** out) {

@@ -339,7 +339,7 @@ int tfhe_gpu_key_create_lagrange(const tfhe_flat_params* p, const double* lagran
 int tfhe_gpu_key_free(tfhe_gpu_key* key);
 int tfhe_gpu_key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 /* split point of the SPLIT32 key: key = hi * 2^split + lo.  Applies to keys created afterwards;
- * 0 restores the default, 24 for N = 1024 and 26 for N = 2048. */
+ * 0 restores the default, 24 for N = 1024 and 25 for N = 2048. */
 int tfhe_gpu_set_split_bits(int32_t bits);
 /* precision used by the upstream-API entry points (boots*, tfhe_bootstrap_FFT, ...) */
 int tfhe_gpu_set_default_precision(int32_t prec);
