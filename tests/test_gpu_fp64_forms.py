@@ -45,10 +45,11 @@ def test_blind_rotate_fp64_forms_match_oracle(gpu_env, forms, shared):
         assert np.array_equal(got[g], O.blind_rotate(keys, acc[g], bara[g])), g
 
 
-@pytest.mark.parametrize("cnt", [8 * 5 + 3, 1100])
+@pytest.mark.parametrize("cnt", [8 * 5 + 3, 256, 257, 1100])
 def test_fp64_forms_agree_on_gates(gpu_env, forms, cnt):
-    """Both forms forced, and the default choice by size (one per workgroup up to 4 per CU, key
-    sharing above), give identical outputs."""
+    """Both forms forced, and the default choice by size (latency form up to one bootstrap per CU,
+    one per workgroup up to 4 per CU, key sharing above), give identical outputs; 256 and 257 NANDs
+    sit on either side of the latency form's limit on a 256-CU device."""
     ks, dk, keys = gpu_env
     rng = np.random.default_rng(78)
     bits = rng.integers(0, 2, (3, cnt)).astype(np.int32)

@@ -63,6 +63,17 @@ def test_empty_and_ragged_batches(gpu_env, rng):
         assert np.array_equal(ks.decrypt(out), bits[0] & bits[1]), cnt
 
 
+@pytest.mark.parametrize("cnt", [256, 257])
+def test_split_latency_form_boundary(gpu_env, cnt, rng):
+    """Split FP32 NANDs on either side of the latency form's limit (one bootstrap per CU on a 256-CU
+    device: the latency form, then the one-wavefront kernel) all decrypt correctly."""
+    ks, dk, _ = gpu_env
+    bits = rng.integers(0, 2, (2, cnt)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    out = dk.gate("NAND", ca, cb, prec=1)
+    assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
+
+
 def test_full_batch_16384_both_paths(gpu_env, rng):
     """The headline workload's size (16384 NANDs in one launch, device-resident ciphertexts):
     every output decrypts to the plaintext NAND on both paths, the host-array entry point returns
