@@ -180,10 +180,11 @@ def adders_report(T, dk, ks, count, reps=2):
     return res
 
 
-def n2048_report(T, gates, steps=2):
+def n2048_report(T, gates, steps=2, audit_levels=12):
     """N=2048 set (n=750, l=3, Bgbit=8, ks t=10): `gates` MUX and `gates` XOR on random bits per
-    step (one launch each), both paths; decryption failures counted over every step, and the split
-    path's external-product error against the FP64 path."""
+    step (one launch each), both paths; decryption failures counted over every step and over a
+    chained audit of `audit_levels` levels of 16384 NANDs per path, and the split path's
+    external-product error against the FP64 path."""
     t = time.perf_counter()
     ks = T.KeySet(2048, seed=(0,))
     dk = T.DeviceKey.from_keyset(ks)
@@ -224,6 +225,20 @@ def n2048_report(T, gates, steps=2):
             res[name]["single_gate_ms"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
             res[name]["single_gate_correct"] = bool(ks.decrypt(o1)[0] == 1 - (bits[0][0] & bits[1][0]))
             log(f"[bench] N=2048 {name}: {res[name]}")
+        # chained audit: levels of 16384 NANDs whose inputs are the previous level's outputs
+        # (bootstrapped noise), every output decrypted
+        for name, prec in (("split32", T.PREC_SPLIT32), ("fp64", T.PREC_FP64)):
+            arng = np.random.default_rng(11)
+            ab = arng.integers(0, 2, 16384).astype(np.int32)
+            act = ks.encrypt(ab)
+            fails = 0
+            for _ in range(audit_levels):
+                perm = arng.permutation(ab.size)
+                act = dk.gate("NAND", act, act[perm], prec=prec)
+                ab = 1 - (ab & ab[perm])
+                fails += int((ks.decrypt(act) != ab).sum())
+            res[name]["audit"] = {"gates": audit_levels * ab.size, "failures": fails, "chained": True}
+            log(f"[bench] N=2048 audit {name}: {res[name]['audit']}")
         acc = rng.integers(-2**31, 2**31, (32, 2, 2048), dtype=np.int64).astype(np.int32)
         rows = rng.integers(0, nn, 32).astype(np.int32)
         r64 = dk.extern_product(acc, rows, prec=T.PREC_FP64)
