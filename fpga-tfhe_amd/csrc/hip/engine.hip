@@ -1577,7 +1577,7 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
             const C2<double>* k = kr + (size_t)((q * L + lvl) * 2) * M + wave * W2_H + lane;
             // the row's first EK key values are requested before the transform, whose length hides
             // their latency (more spill at 256 VGPRs); the rest after it
-            constexpr int EK = 2;
+            constexpr int EK = 3;
             C2<double> k0[EK];
 #pragma unroll
             for (int r = 0; r < EK; ++r) k0[r] = k[r * 64];
