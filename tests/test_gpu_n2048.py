@@ -297,3 +297,33 @@ def test_circuit_n2048_adder_graph_replay(env2048, prec):
         d_in.free()
         d_out.free()
         c.free()
+
+
+def test_split_point_override_n2048(env2048):
+    """tfhe_gpu_set_split_bits applies to keys created afterwards: a key made at a 26-bit split
+    gives external products within the stated bound, about twice the default 25-bit split's error,
+    and correct gates; 0 restores the default for later keys."""
+    ks, dk, keys = env2048
+    rng = np.random.default_rng(2070)
+    acc = rand_torus(rng, (3, 2, N))
+    rows = np.array([2, 333, 747], np.int32)
+    T().set_split_bits(26)
+    try:
+        dk26 = T().DeviceKey.from_keyset(ks)
+    finally:
+        T().set_split_bits(0)
+    try:
+        rms = {}
+        for name, key in (("26", dk26), ("default", dk)):
+            got = key.extern_product(acc, rows, prec=1)
+            e = np.concatenate([np.abs((got[g].astype(np.int64) - O.extern_product_exact(P, acc[g], keys.bk[rows[g]])
+                                        .astype(np.int64) + 2**31) % 2**32 - 2**31).ravel() for g in range(3)])
+            rms[name] = float(np.sqrt((e.astype(float) ** 2).mean()))
+            print(f"N=2048 split {name}: external product error max {e.max()} rms {rms[name]:.1f}", flush=True)
+            assert e.max() < 2**18
+        assert 1.5 < rms["26"] / rms["default"] < 2.7
+        bits = rng.integers(0, 2, (2, 24)).astype(np.int32)
+        out = dk26.gate("NAND", ks.encrypt(bits[0]), ks.encrypt(bits[1]), prec=1)
+        assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
+    finally:
+        dk26.free()
