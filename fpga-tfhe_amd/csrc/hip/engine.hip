@@ -1830,22 +1830,29 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                     }
                 }
             } else {
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
+                // the two rows pipelined as on the split path: the first key polynomial before the
+                // transform, then EK1 of the second polynomial's values before polynomial 1's
+                // rotation and digits (all eight fit too, but ran slower), the rest after them
+                constexpr int EK1 = 4;
                 uint32_t u[NC];
+                auto rotate = [&](int q) {
 #pragma unroll
-                for (int r = 0; r < NC; ++r)
-                    u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
-                C2<T> x[R1];
-                {
+                    for (int r = 0; r < NC; ++r)
+                        u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
+                };
+                auto prep = [&](C2<double>* x) {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa) {
                         const C2<double> p0{(double)digit_sx<BG>(u[aa], lvl), (double)digit_sx<BG>(u[aa + 16], lvl)};
                         const C2<double> p1{(double)digit_sx<BG>(u[aa + 8], lvl), (double)digit_sx<BG>(u[aa + 24], lvl)};
                         x[aa] = w2_prep(p0, p1, rw64[aa], v, aa == 0);
                     }
-                    // the first key polynomial is requested before the transform, the second after
-                    // it (FP64 has no room to pipeline the rows as the split path does)
+                };
+                C2<double> x[R1];
+                rotate(0);
+                prep(x);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
                     const C2<double>* kp = (const C2<double>*)A.bk + (size_t)i * ROWSET +
                                            (size_t)((q * L + lvl) * 2) * M + v * W2_H + lane;
                     C2<double> k0[R1];
@@ -1854,10 +1861,21 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                     fft_fwd<R1, double, true, false>(x, twl, s_x[w], lane);
 #pragma unroll
                     for (int r = 0; r < R1; ++r) cmac(acc[0][r], x[r], k0[r]);
+                    C2<double> k1[EK1];
 #pragma unroll
-                    for (int r = 0; r < R1; ++r) cmac(acc[1][r], x[r], kp[M + r * 64]);
+                    for (int r = 0; r < EK1; ++r) k1[r] = kp[M + r * 64];
+                    C2<double> xn[R1];
+                    if (q == 0) {
+                        rotate(1);
+                        prep(xn);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) cmac(acc[1][r], x[r], r < EK1 ? k1[r < EK1 ? r : 0] : kp[M + r * 64]);
+                    if (q == 0) {
+#pragma unroll
+                        for (int r = 0; r < R1; ++r) x[r] = xn[r];
+                    }
                 }
-            }
             }
 #pragma unroll
             for (int k = 0; k < K; ++k)
