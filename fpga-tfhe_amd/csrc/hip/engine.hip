@@ -1467,11 +1467,10 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
             cmac(h1[r], v[r], C2<float>{k1[r].x, k1[r].y});
             cmac(l1[r], v[r], C2<float>{k1[r].z, k1[r].w});
         }
-        // issued after this row's key loads (vmcnt drains in issue order), so it overlaps the
-        // next transform and gates nothing but the next row's loads
+        // a scheduling barrier between rows.  The next row's warm-up loads that used to sit here
+        // cost 4.5% once the key requests moved ahead of the transforms; without the barrier the
+        // step is 6% slower.  (The FP64 step keeps its warm-up loads: 30% slower without.)
         __builtin_amdgcn_sched_barrier(0);
-        w2_warm_row(row + 1 < 2 * L ? kr + (size_t)((row + 1) * 2) * M + wave * W2_H : kr_next + wave * W2_H, lane,
-                    sink);
         if (row + 1 < 2 * L) {
 #pragma unroll
             for (int r = 0; r < R1; ++r) v[r] = vn[r];
