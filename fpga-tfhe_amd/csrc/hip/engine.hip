@@ -1398,8 +1398,7 @@ __device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, floa
 // the transform exchange buffer (C2<float>) and two halves for the output exchange.
 template <int L, int BG, bool ROT>
 __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const void* sx_other, int a,
-                                         const float4* __restrict__ kr, const float4* __restrict__ kr_next,
-                                         const LaneTw<float, 8>& tw, const C2<float>* __restrict__ rw32,
+                                         const float4* __restrict__ kr, const LaneTw<float, 8>& tw, const C2<float>* __restrict__ rw32,
                                          const C2<double>* __restrict__ rws, int wave, int lane, int split_bits,
                                          double* raw, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
@@ -1412,7 +1411,6 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
         h0[r] = C2<float>{0.f, 0.f}; l0[r] = C2<float>{0.f, 0.f};
         h1[r] = C2<float>{0.f, 0.f}; l1[r] = C2<float>{0.f, 0.f};
     }
-    float sink = 0.f;
     // an idle bootstrap (zero rotation, or past the batch end) skips the arithmetic but still takes
     // part in every workgroup barrier below
     if (work) {
@@ -1476,7 +1474,6 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
             for (int r = 0; r < R1; ++r) v[r] = vn[r];
         }
     }
-    asm volatile("" ::"v"(sink));
     fft_inv<R1>(h0, tw, sbuf, lane);
     fft_inv<R1>(l0, tw, sbuf, lane);
     fft_inv<R1>(h1, tw, sbuf, lane);
@@ -1669,15 +1666,14 @@ k_boot2w(BootKArgs A) {
         const float4* key = (const float4*)A.bk;
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
-            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw32, rws, wave, lane,
+            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, tw, rw32, rws, wave, lane,
                                    A.split_bits, raw, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
                 // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); a is the same in both
                 // wavefronts of a bootstrap, and an idle bootstrap still meets every barrier
                 const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
-                step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
-                                      key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw32, rws, wave, lane,
+                step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET, tw, rw32, rws, wave, lane,
                                       A.split_bits, nullptr, a != 0);
             }
         }
