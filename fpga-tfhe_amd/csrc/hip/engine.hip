@@ -1538,19 +1538,20 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
                                          const C2<double>* __restrict__ kr, const C2<double>* __restrict__ kr_next,
                                          const LaneTwS<double, 8>& tw0,
                                          const C2<double>* __restrict__ rw_own, const C2<double>* __restrict__ rw_other,
-                                         int wave, int lane, double* raw, bool work) {
+                                         int wave, int lane, double* raw, bool work, float& sink) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     // an offset the compiler cannot see through keeps the pass-2 twiddle reads at their uses (held
-    // across steps they would cost 28 registers)
+    // across steps they would cost 28 registers).  The statement is not volatile, so it is no memory
+    // clobber and the uniform constant loads after it can be scalar loads (0.5% per step); its input
+    // a changes from step to step, so it is not hoisted out of the step loop.
     int t2o = 0;
-    asm volatile("" : "+v"(t2o));
+    asm("" : "+v"(t2o) : "s"(a));
     LaneTwS<double, 8> tw = tw0;
     tw.t2 = tw0.t2 + t2o;
     C2<double> o0[R1], o1[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) { o0[r] = C2<double>{0.0, 0.0}; o1[r] = C2<double>{0.0, 0.0}; }
-    float sink = 0.f;
     if (work) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -1587,7 +1588,6 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
                         sink);
         }
     }
-    asm volatile("" ::"v"(sink));
     fft_inv<R1, double, false, false>(o0, tw, sx_own, lane);
     fft_inv<R1, double, false, false>(o1, tw, sx_own, lane);
     }
@@ -1685,18 +1685,22 @@ k_boot2w(BootKArgs A) {
         C2<double>* sx_own = (C2<double>*)s_xd[b][wave];
         const C2<double>* sx_other = (const C2<double>*)s_xd[b][1 - wave];
         const C2<double>* key = (const C2<double>*)A.bk;
+        // the warm-up loads are folded into sink, which is consumed once after the last step (an
+        // empty asm statement inside the step would stop uniform loads from becoming scalar loads)
+        float sink = 0.f;
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
             step2w64<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, key + rs, tw, rw_own, rw_other, wave, lane,
-                                   raw, active);
+                                   raw, active, sink);
         } else {
             for (int i = 0; i < A.n; ++i) {
                 const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
                 step2w64<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET,
                                       key + (size_t)(i + 1 < A.n ? i + 1 : i) * ROWSET, tw, rw_own, rw_other, wave,
-                                      lane, nullptr, a != 0);
+                                      lane, nullptr, a != 0, sink);
             }
         }
+        asm volatile("" ::"v"(sink));
     }
     __syncthreads();
     if (active && wave == 0) boot_epilogue<N>(A, g, lane, s_acc);
