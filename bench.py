@@ -132,7 +132,18 @@ def precision_report(T, dk, rng, count=64):
     hist = np.bincount(np.ceil(np.log2(err.astype(np.float64) + 1)).astype(int), minlength=1)
     return {"samples": int(err.size), "max": int(err.max()), "rms": round(float(np.sqrt((err.astype(float) ** 2).mean())), 1),
             "unit": "2^-32 torus", "log2_histogram": {f"<=2^{i}": int(c) for i, c in enumerate(hist) if c},
-            "fp64_max_dist_to_integer": dist}
+            "fp64_max_dist_to_integer": dist, "split_bits": dk.split_bits,
+            "split_hi_rounding": hi_margin(dk, acc, rows)}
+
+
+def hi_margin(dk, acc, rows):
+    """The split path is exact while the digits x hi-key-half product, before rounding, stays within
+    1/2 of its integer value (a slip would cost 2^split units at once): its largest and rms distance
+    to the nearest integer over the sample."""
+    _, hi = dk.extern_product_split_hi(acc, rows)
+    d = np.abs(hi - np.rint(hi))
+    return {"max_dist_to_integer": round(float(d.max()), 5), "rms_dist": round(float(np.sqrt((d ** 2).mean())), 5),
+            "samples": int(d.size), "slip_at": 0.5}
 
 
 def adders_report(T, dk, ks, count, reps=2):
@@ -241,12 +252,15 @@ def n2048_report(T, gates, steps=2, audit_levels=12):
             log(f"[bench] N=2048 audit {name}: {res[name]['audit']}")
         acc = rng.integers(-2**31, 2**31, (32, 2, 2048), dtype=np.int64).astype(np.int32)
         rows = rng.integers(0, nn, 32).astype(np.int32)
-        r64 = dk.extern_product(acc, rows, prec=T.PREC_FP64)
+        r64, raw64 = dk.extern_product(acc, rows, prec=T.PREC_FP64, want_raw=True)
         r32 = dk.extern_product(acc, rows, prec=T.PREC_SPLIT32)
         err = np.abs(((r32.astype(np.int64) - r64.astype(np.int64) + 2**31) % 2**32) - 2**31)
         res["split32_extern_product_error"] = {"samples": int(err.size), "max": int(err.max()),
                                                "rms": round(float(np.sqrt((err.astype(float) ** 2).mean())), 1),
-                                               "unit": "2^-32 torus"}
+                                               "unit": "2^-32 torus",
+                                               "fp64_max_dist_to_integer": float(np.abs(raw64 - np.rint(raw64)).max()),
+                                               "split_bits": dk.split_bits,
+                                               "split_hi_rounding": hi_margin(dk, acc, rows)}
     finally:
         for b in bufs + outs:
             b.free()
@@ -255,11 +269,46 @@ def n2048_report(T, gates, steps=2, audit_levels=12):
     return res
 
 
+def host_cpu():
+    """Host CPU as the box presents it: logical CPUs, the CPUs this process may run on, the OpenMP
+    thread limit set in the environment, and the model name (lscpu, else /proc/cpuinfo)."""
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = info["nproc"]
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    info["omp_num_threads"] = int(omp) if omp.isdigit() and int(omp) > 0 else None
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    if not model:
+        try:
+            model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")][0]
+        except Exception:
+            model = "unknown"
+    info["model"] = model
+    return info
+
+
 def cpu_baseline(ks, batch_bits, seconds):
+    """The oracle's FP64 FFT gate path (bootstrap + keyswitch per gate, one gate per OpenMP thread)
+    on every host CPU this process may use -- the affinity set, capped by OMP_NUM_THREADS where the
+    environment sets it -- and on one thread, over bounded samples of the same NAND workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     keys = O.Keys(O.PARAMS_1024, ks.lwe_key, ks.tlwe_key, ks.ksk, ks.bk)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = host["affinity"]
+    if host["omp_num_threads"]:
+        threads = min(threads, host["omp_num_threads"])
+    threads = max(1, threads)
     try:
         cal = 2 * threads
         ca, cb = ks.encrypt(batch_bits[0][:cal]), ks.encrypt(batch_bits[1][:cal])
@@ -273,12 +322,10 @@ def cpu_baseline(ks, batch_bits, seconds):
         one = max(1, int(seconds / 5 * cnt / sec / threads))
         out1, sec1 = O.gate_batch(keys, "NAND", ca[:one], cb[:one], nthreads=1)
         ok = ok and bool(np.array_equal(out1, out[:one]))
-        try:
-            model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")][0]
-        except Exception:
-            model = "unknown"
         return {"value": cnt / sec, "unit": "gates/s", "cores": threads, "kind": "port",
-                "sample": f"{cnt} NAND gates (bootstrap + keyswitch, FP64 FFT path), {sec:.1f} s, {model}",
+                "sample": f"{cnt} NAND gates (bootstrap + keyswitch, FP64 FFT path), {sec:.1f} s on {threads} threads, "
+                          f"{host['model']}",
+                "host": host,
                 "single_thread": {"value": one / sec1, "unit": "gates/s", "sample": f"{one} of the same gates, {sec1:.1f} s"},
                 "correct": ok}
     finally:
@@ -372,10 +419,17 @@ def main():
     hr = res[head]
     achieved = FLOP_PER_GATE * B / (hr["boot_ms_avg"] * 1e-3) / 1e12
     traffic = None
+    # every bootstrap streams the whole FFT-domain key (61.9 MB at N=1024) once: the demand on the
+    # cache hierarchy, measured live; the profile adds what L2 and HBM actually served
+    key_bytes = n * (k + 1) * l * (k + 1) * M * 16
+    key_stream = {"bytes_per_bootstrap": key_bytes, "bytes_per_launch": key_bytes * B,
+                  "demand_gbps": round(key_bytes * B / (hr["boot_ms_avg"] * 1e-3) / 1e9, 1)}
     tpath = os.path.join(ROOT, "profiles", f"traffic_bootstrap_{head}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        key_stream["profiled"] = {kk: prof.get(kk) for kk in ("l2_to_l1_gbps", "l2_hit_rate", "hbm_gbps", "avg_ms", "source")}
     line = {
         "metric": METRIC,
         "value": round(hr["gates_per_s"], 1),
@@ -401,10 +455,14 @@ def main():
         "single_gate_latency": single or None,
         "audit": aud or None,
         "split32_extern_product_error": prec_rep,
-        "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 4),
                      "traffic": traffic, "kernel": f"bootstrap_{head}", "flop_per_gate": FLOP_PER_GATE,
-                     "measured_valu_peak": {"fp64": round(vp64, 1), "fp32_unpacked": round(vp32, 1)}},
+                     "bound_note": "vector-ALU FLOP roofline: the transforms and the pointwise MAC run on the VALU, "
+                                   "no matrix-core instructions; peak is the published FP32 (split path) or FP64 "
+                                   "vector rate",
+                     "measured_valu_peak": {"fp64": round(vp64, 1), "fp32_unpacked": round(vp32, 1)},
+                     "key_stream": key_stream},
         "cpu_baseline": cpu,
         "circuits_32bit_adders": adders,
         "n2048": n2048,

@@ -52,7 +52,7 @@ struct LinDesc {
 //           (g' = g - seg_start[s]), modulus-switches it, builds the X^{-barb} test vector of mu,
 //           blind-rotates and writes the extracted sample u_out[g][0..N].
 //   mode 1: acc_io[g] (k+1 = 2 polys) is blind-rotated by bara_in[g][0..n) in place.
-//   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out optional.
+//   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out, raw_hi optional.
 //   mode 3: as mode 0, with the gate inputs of job g taken from segs[g / seg_jobs].
 // form (FP64, N=1024): 0 = the process-wide choice, 1 = key-sharing workgroups, 2 = one bootstrap
 // per workgroup, 3 = one bootstrap per workgroup of 2l wavefronts (latency form; modes 0, 1, 3).
@@ -72,6 +72,7 @@ struct BootLaunch {
     const int32_t* bara_in = nullptr;
     const int32_t* row_set = nullptr;
     double* raw_out = nullptr;
+    double* raw_hi = nullptr;   // mode 2, split FP32: the hi product before rounding (units of 2^split)
     int32_t* u_out = nullptr;
 };
 

@@ -107,7 +107,7 @@ __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lan
         uint32_t xbv = (uint32_t)cst + al * (uint32_t)xa[n];
         if (xb) xbv += be * (uint32_t)xb[n];
         const int barb = (int)((xbv + (1u << (SH - 1))) >> SH);
-        // b = X^{2N - barb} [mu, ..., mu] (mulByXaiOrCopy.cpp:310-326), a = 0 (tb :317)
+        // b = X^{2N - barb} [mu, ..., mu] (mulByXaiOrCopy.cpp:25-41), a = 0 (tb :317)
         const int sft = (2 * N - barb) & (2 * N - 1);
         for (int j = lane; j < N; j += 64) {
             s_acc[j] = 0;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void boot_epilogue(const BootKArgs& A, int g, int lan
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP64 path: one MuxRotate step (tfhe_MuxRotate_FFT.cpp:101-130) or one plain external product.
+// FP64 path: one MuxRotate step (tfhe_MuxRotate_FFT.cpp:56-85) or one plain external product.
 //   ROT:  acc <- acc + bk_i [.] ((X^a - 1) acc)
 //   !ROT: acc <- bk_i [.] acc
 // Decomposition and rotation are exact integer work; the external product runs the folded FP64
@@ -275,7 +275,7 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
             // prefetch unconditional
             const C2<double>* knext = key + (size_t)(i + 1 < n ? i + 1 : i) * ROWSET;
             if (a == 0) {
-                // zero rotation: the step is skipped (tfhe_blindRotate_FFT.cpp:181); the buffer must
+                // zero rotation: the step is skipped (tfhe_blindRotate_FFT.cpp:114); the buffer must
                 // then hold row 0 of the next key instead (rare: one step in 2N)
                 __builtin_amdgcn_s_waitcnt(0);
                 key_row_to_lds<M>(s_key, knext, 0, lane);
@@ -553,7 +553,8 @@ __device__ __forceinline__ float4 key_load(__amdgpu_buffer_rsrc_t rs, uint32_t l
 template <int N, int L, int BG, bool ROT>
 __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, __amdgpu_buffer_rsrc_t kr,
                                        uint32_t row0, const DitTw<float>& tw, const C2<float>* tws,
-                                       const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw) {
+                                       const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw,
+                                       double* raw_hi) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
     static_assert(M == 512, "the split path runs the 512-point decimation-in-time transform");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
@@ -619,6 +620,10 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             raw[j] = zh0.x * hs + zl0.x; raw[j + M] = zh0.y * hs + zl0.y;
             raw[N + j] = zh1.x * hs + zl1.x; raw[N + j + M] = zh1.y * hs + zl1.y;
         }
+        if (raw_hi) {   // the hi product before rounding, in units of 2^split_bits
+            raw_hi[j] = zh0.x; raw_hi[j + M] = zh0.y;
+            raw_hi[N + j] = zh1.x; raw_hi[N + j + M] = zh1.y;
+        }
         if (ROT) {
             s_acc[j] = (int32_t)((uint32_t)s_acc[j] + r00);
             s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + r01);
@@ -658,14 +663,16 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
 
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+        double* raw_hi = A.L.raw_hi ? A.L.raw_hi + (size_t)g * 2 * N : nullptr;
         step32<N, L, BG, false>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, untw, lane, sb,
-                                raw);
+                                raw, raw_hi);
     } else {
         const int n = A.n;
         for (int i = 0; i < n; ++i) {
             const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
             if (a == 0) continue;
-            step32<N, L, BG, true>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb, nullptr);
+            step32<N, L, BG, true>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb, nullptr,
+                                   nullptr);
         }
     }
     boot_epilogue<N>(A, g, lane, s_acc);
@@ -1400,7 +1407,7 @@ template <int L, int BG, bool ROT>
 __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const void* sx_other, int a,
                                          const float4* __restrict__ kr, const LaneTw<float, 8>& tw, const C2<float>* __restrict__ rw32,
                                          const C2<double>* __restrict__ rws, int wave, int lane, int split_bits,
-                                         double* raw, bool work) {
+                                         double* raw, double* raw_hi, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float>* sbuf = (C2<float>*)sx_own;
@@ -1511,6 +1518,10 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                     rh[2 * aa] = round_mod32(z.x);
                     rh[2 * aa + 1] = round_mod32(z.y);
                     ph[aa] = z;
+                    if (raw_hi) {   // the hi product before rounding, in units of 2^split_bits
+                        raw_hi[q * N + w2_coef(wave, lane, aa, 0)] = z.x;
+                        raw_hi[q * N + w2_coef(wave, lane, aa, 1)] = z.y;
+                    }
                 } else {
                     const uint32_t c0 = (rh[2 * aa] << split_bits) + round_mod32(z.x);
                     const uint32_t c1 = (rh[2 * aa + 1] << split_bits) + round_mod32(z.y);
@@ -1666,15 +1677,16 @@ k_boot2w(BootKArgs A) {
         const float4* key = (const float4*)A.bk;
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
+            double* raw_hi = (A.L.raw_hi && active) ? A.L.raw_hi + (size_t)g * 2 * N : nullptr;
             step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, tw, rw32, rws, wave, lane,
-                                   A.split_bits, raw, active);
+                                   A.split_bits, raw, raw_hi, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
                 // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); a is the same in both
                 // wavefronts of a bootstrap, and an idle bootstrap still meets every barrier
                 const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
                 step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET, tw, rw32, rws, wave, lane,
-                                      A.split_bits, nullptr, a != 0);
+                                      A.split_bits, nullptr, nullptr, a != 0);
             }
         }
     } else {

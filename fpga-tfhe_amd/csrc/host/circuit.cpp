@@ -59,7 +59,10 @@ struct tfhe_circuit {
     std::vector<int32_t> consts;
     int64_t n_boot = 0, max_level_boot = 0;
     // ---- binding of the last run ----
+    // the bound key: its address and creation number (the address alone can be reused by a key
+    // created after this one is freed, and a captured graph holds the old key's device pointers)
     tfhe_gpu_key* key = nullptr;
+    uint64_t key_id = 0;
     int32_t prec = -1, batch = 0, flags = 0;
     const int32_t* d_in = nullptr;
     int32_t* d_out = nullptr;
@@ -220,6 +223,7 @@ int bind(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec, const int32_t* d_in, 
     if ((e = hipStreamSynchronize(lib_stream())) != hipSuccess) return lib_hip_fail("circuit tables sync", e);
     release_binding(c);
     c->key = key;
+    c->key_id = key_id(key);
     c->prec = prec;
     c->batch = batch;
     c->d_in = d_in;
@@ -424,7 +428,7 @@ extern "C" int tfhe_circuit_run(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec
     if ((rc = lib_twiddles(kv.p.N, &tw))) return rc;
     if ((rc = plan(c))) return rc;
     if ((int64_t)c->max_level_boot * batch > (int64_t)INT32_MAX / 2) return lib_arg_fail("level too wide for one launch");
-    const bool rebind = !c->bound || c->key != key || c->prec != prec || c->batch != batch || c->d_in != d_in ||
+    const bool rebind = !c->bound || c->key != key || c->key_id != key_id(key) || c->prec != prec || c->batch != batch || c->d_in != d_in ||
                         c->d_out != d_out;
     if (rebind && (rc = bind(c, key, prec, d_in, d_out, batch))) return rc;
     const bool tune = (flags & TFHE_CIRCUIT_AUTOTUNE) != 0;

@@ -58,6 +58,8 @@ int lib_hip_fail(const char* what, hipError_t e);
 int lib_arg_fail(const char* what);
 int key_check_prec(tfhe_gpu_key* key, int32_t prec);
 DevKeyView key_view(const tfhe_gpu_key* key);
+// creation number of a device key (unique in the process; never reused after the key is freed)
+uint64_t key_id(const tfhe_gpu_key* key);
 // gate pre-combination x = (0, C) + alpha a + beta b of the two-input gates
 void gate_coefficients(int gate, int32_t* C, int32_t* alpha, int32_t* beta);
 // per-launch timing under tfhe_gpu_profile_enable (a no-op otherwise)

@@ -19,6 +19,9 @@
 using namespace tfhe;
 
 struct tfhe_gpu_key {
+    // creation number, unique for the life of the process: a circuit bound to a key remembers it,
+    // so a key created later at the same host address is not mistaken for the freed one
+    uint64_t id = 0;
     DevParams p{};
     void* bk64 = nullptr;
     void* bk32 = nullptr;
@@ -358,6 +361,7 @@ int lib_hip_fail(const char* what, hipError_t e) { return hip_fail(what, e); }
 int lib_arg_fail(const char* what) { return arg_fail(what); }
 int key_check_prec(tfhe_gpu_key* key, int32_t prec) { return check_key_prec(key, prec); }
 DevKeyView key_view(const tfhe_gpu_key* key) { return key->view(); }
+uint64_t key_id(const tfhe_gpu_key* key) { return key->id; }
 void gate_coefficients(int gate, int32_t* C, int32_t* alpha, int32_t* beta) { gate_coeffs(gate, C, alpha, beta); }
 
 LibProf::LibProf(const char* name) : impl(new ProfScope(ctx(), name)) {}
@@ -443,6 +447,8 @@ extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk
     DevTwiddles* tw;
     if ((rc = ensure_twiddles(p.N, &tw))) return rc;
     tfhe_gpu_key* k = new tfhe_gpu_key();
+    static uint64_t next_id = 0;
+    k->id = ++next_id;   // under the library mutex
     k->p = p;
     k->split_bits = c.split_bits ? c.split_bits : split_bits_for(p);
     const int M = p.N / 2, kpl = 2 * p.l;
@@ -613,7 +619,8 @@ extern "C" int tfhe_gpu_boots_batch(int32_t gate, LweSample* out, const LweSampl
 // ---- stage entry points (host arrays) ----
 
 static int boot_stage(tfhe_gpu_key* key, int32_t prec, int mode, int32_t* io, const int32_t* x, const int32_t* bara,
-                      const int32_t* row_set, double* raw, int32_t mu, int32_t count, int32_t* u_host) {
+                      const int32_t* row_set, double* raw, int32_t mu, int32_t count, int32_t* u_host,
+                      double* raw_hi = nullptr) {
     Ctx& cx = ctx();
     int rc = ensure_init();
     if (rc) return rc;
@@ -627,7 +634,7 @@ static int boot_stage(tfhe_gpu_key* key, int32_t prec, int mode, int32_t* io, co
     L.mode = mode;
     L.count = count;
     L.mu = mu;
-    void *dx = nullptr, *dacc = nullptr, *dbara = nullptr, *drow = nullptr, *draw = nullptr;
+    void *dx = nullptr, *dacc = nullptr, *dbara = nullptr, *drow = nullptr, *draw = nullptr, *draw_hi = nullptr;
     const size_t acc_bytes = (size_t)count * 2 * N * sizeof(int32_t);
     if (mode == 0) {
         const size_t xb = (size_t)count * (n + 1) * sizeof(int32_t);
@@ -660,6 +667,10 @@ static int boot_stage(tfhe_gpu_key* key, int32_t prec, int mode, int32_t* io, co
                 if ((rc = key_io_buffer(key, 4, acc_bytes * 2, &draw))) return rc;
                 L.raw_out = (double*)draw;
             }
+            if (raw_hi) {
+                if ((rc = key_io_buffer(key, 5, acc_bytes * 2, &draw_hi))) return rc;
+                L.raw_hi = (double*)draw_hi;
+            }
         }
     }
     {
@@ -673,6 +684,7 @@ static int boot_stage(tfhe_gpu_key* key, int32_t prec, int mode, int32_t* io, co
     } else {
         if ((rc = d2h_sync(io, dacc, acc_bytes))) return rc;
         if (raw && (rc = d2h_sync(raw, draw, acc_bytes * 2))) return rc;
+        if (raw_hi && (rc = d2h_sync(raw_hi, draw_hi, acc_bytes * 2))) return rc;
     }
     return TFHE_OK;
 }
@@ -724,6 +736,19 @@ extern "C" int tfhe_gpu_extern_product_batch(tfhe_gpu_key* key, int32_t prec, in
     std::lock_guard<std::recursive_mutex> g(ctx().mu);
     if (count > 0 && (!acc || !row_set)) return arg_fail("null argument");
     return boot_stage(key, prec, 2, acc, nullptr, nullptr, row_set, raw, 0, count, nullptr);
+}
+
+extern "C" int tfhe_gpu_extern_product_split_hi_batch(tfhe_gpu_key* key, int32_t* acc, const int32_t* row_set,
+                                                      double* raw_hi, int32_t count) {
+    std::lock_guard<std::recursive_mutex> g(ctx().mu);
+    if (count > 0 && (!acc || !row_set || !raw_hi)) return arg_fail("null argument");
+    return boot_stage(key, TFHE_PREC_SPLIT32, 2, acc, nullptr, nullptr, row_set, nullptr, 0, count, nullptr, raw_hi);
+}
+
+extern "C" int tfhe_gpu_key_split_bits(const tfhe_gpu_key* key, int32_t* bits) {
+    if (!key || !bits) return arg_fail("null argument");
+    *bits = key->split_bits;
+    return TFHE_OK;
 }
 
 extern "C" int tfhe_gpu_keyswitch_batch(tfhe_gpu_key* key, int32_t* out, const int32_t* u, int32_t count) {

@@ -769,7 +769,7 @@ extern "C" void tfhe_blindRotateAndExtract_FFT(LweSample* result, const TorusPol
     const TLweParams* accp = bk_params->tlwe_params;
     const int N = accp->N;
     TLweSample* acc = new_TLweSample(accp);
-    // testvectbis = X^{2N - barb} v  (mulByXaiOrCopy.cpp:310-326), acc = (0, testvectbis)
+    // testvectbis = X^{2N - barb} v  (mulByXaiOrCopy.cpp:25-41), acc = (0, testvectbis)
     const int s = (2 * N - barb) % (2 * N);
     for (int j = 0; j < N; ++j) {
         const int e = ((j - s) % (2 * N) + 2 * N) % (2 * N);

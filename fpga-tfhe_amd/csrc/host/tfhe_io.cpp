@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
 #include <mutex>
 #include <vector>
 
@@ -28,6 +29,8 @@ using namespace tfhe;
 namespace {
 
 constexpr uint32_t kVersion = 1;
+constexpr int kMaxKsBasebit = 8;
+constexpr uint64_t kMaxKeyElems = 1ull << 30;
 
 std::mutex& reg_mutex() {
     static std::mutex m;
@@ -103,9 +106,18 @@ TFheGateBootstrappingParameterSet* read_params(Reader& r) {
     f.ks_basebit = r.i32();
     const double a0 = r.f64(), a1 = r.f64(), a2 = r.f64(), a3 = r.f64();
     if (!r.ok) return nullptr;
-    // sizes a file may describe: anything the host containers can hold sensibly
+    // sizes a file may describe: anything the host containers can hold sensibly.  The keyswitch
+    // base 2^basebit is bounded before it is formed, and so are the element counts of the two keys
+    // the file would then bring (kMaxKeyElems int32 each, 4 GiB), before anything is allocated.
     if (f.n <= 0 || f.n > 4096 || f.N <= 0 || f.N > 65536 || (f.N & (f.N - 1)) || f.k < 1 || f.k > 4 || f.l < 1 ||
-        f.l > 16 || f.Bgbit < 1 || f.l * f.Bgbit > 32 || f.ks_t < 1 || f.ks_basebit < 1 || f.ks_t * f.ks_basebit > 32) {
+        f.l > 16 || f.Bgbit < 1 || f.l * f.Bgbit > 32 || f.ks_t < 1 || f.ks_basebit < 1 ||
+        f.ks_basebit > kMaxKsBasebit || f.ks_t * f.ks_basebit > 32) {
+        r.ok = false;
+        return nullptr;
+    }
+    const uint64_t bk_elems = (uint64_t)f.n * (f.k + 1) * f.l * (f.k + 1) * f.N;
+    const uint64_t ksk_elems = (uint64_t)f.k * f.N * f.ks_t * (1u << f.ks_basebit) * (f.n + 1);
+    if (bk_elems > kMaxKeyElems || ksk_elems > kMaxKeyElems) {
         r.ok = false;
         return nullptr;
     }
@@ -205,7 +217,7 @@ extern "C" void export_tfheGateBootstrappingParameterSet_toFile(FILE* F, const T
     if (!w.ok) fatal("export_tfheGateBootstrappingParameterSet_toFile (write)", TFHE_E_ARG);
 }
 
-extern "C" TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile(FILE* F) {
+static TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile_impl(FILE* F) {
     if (!F) return fail_io("null FILE"), nullptr;
     Reader r{F};
     TFheGateBootstrappingParameterSet* p = read_params(r);
@@ -220,7 +232,7 @@ extern "C" void export_tfheGateBootstrappingCloudKeySet_toFile(FILE* F, const TF
     if (!w.ok) fatal("export_tfheGateBootstrappingCloudKeySet_toFile (write)", TFHE_E_ARG);
 }
 
-extern "C" TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromFile(FILE* F) {
+static TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromFile_impl(FILE* F) {
     if (!F) return fail_io("null FILE"), nullptr;
     Reader r{F};
     if (!r.tag("TMCK")) return fail_io("not a cloud-key object of this library (tag TMCK)"), nullptr;
@@ -249,7 +261,7 @@ extern "C" void export_tfheGateBootstrappingSecretKeySet_toFile(FILE* F, const T
     if (!w.ok) fatal("export_tfheGateBootstrappingSecretKeySet_toFile (write)", TFHE_E_ARG);
 }
 
-extern "C" TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromFile(FILE* F) {
+static TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromFile_impl(FILE* F) {
     if (!F) return fail_io("null FILE"), nullptr;
     Reader r{F};
     if (!r.tag("TMSK")) return fail_io("not a secret-key object of this library (tag TMSK)"), nullptr;
@@ -304,5 +316,38 @@ extern "C" void import_gate_bootstrapping_ciphertext_fromFile(FILE* F, LweSample
     if (!r.ok) {
         set_error("truncated ciphertext file");
         fatal("import_gate_bootstrapping_ciphertext_fromFile", TFHE_E_ARG);
+    }
+}
+
+extern "C" TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile(FILE* F) {
+    // a size the parameter checks admit can still exceed host memory: report it, do not throw
+    // through the C ABI
+    try {
+        return new_tfheGateBootstrappingParameterSet_fromFile_impl(F);
+    } catch (const std::bad_alloc&) {
+        fail_io("out of host memory while reading the file");
+        return nullptr;
+    }
+}
+
+extern "C" TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromFile(FILE* F) {
+    // a size the parameter checks admit can still exceed host memory: report it, do not throw
+    // through the C ABI
+    try {
+        return new_tfheGateBootstrappingCloudKeySet_fromFile_impl(F);
+    } catch (const std::bad_alloc&) {
+        fail_io("out of host memory while reading the file");
+        return nullptr;
+    }
+}
+
+extern "C" TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromFile(FILE* F) {
+    // a size the parameter checks admit can still exceed host memory: report it, do not throw
+    // through the C ABI
+    try {
+        return new_tfheGateBootstrappingSecretKeySet_fromFile_impl(F);
+    } catch (const std::bad_alloc&) {
+        fail_io("out of host memory while reading the file");
+        return nullptr;
     }
 }

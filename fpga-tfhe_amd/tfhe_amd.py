@@ -113,6 +113,8 @@ def lib():
         "tfhe_gpu_keyswitch_batch": (ctypes.c_int, [_vp, _i32p, _i32p, i32]),
         "tfhe_gpu_extract_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32]),
         "tfhe_gpu_extern_product_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, _f64p, i32]),
+        "tfhe_gpu_extern_product_split_hi_batch": (ctypes.c_int, [_vp, _i32p, _i32p, _f64p, i32]),
+        "tfhe_gpu_key_split_bits": (ctypes.c_int, [_vp, P(i32)]),
         "tfhe_gpu_rotate_decompose_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, _i32p, _i32p, i32]),
         "tfhe_gpu_fft_forward_batch": (ctypes.c_int, [i32, i32, _f64p, _i32p, i32]),
         "tfhe_gpu_fft_inverse_batch": (ctypes.c_int, [i32, i32, _f64p, _f64p, i32]),
@@ -338,6 +340,21 @@ class DeviceKey:
         _ck(lib().tfhe_gpu_extern_product_batch(self.ptr, prec, i32p(acc), i32p(rs), f64p(raw), acc.shape[0]),
             "extern_product")
         return (acc, raw) if want_raw else acc
+
+    def extern_product_split_hi(self, acc, row_set):
+        """Split-FP32 external product and the hi-half product before rounding (units of 2^split)."""
+        acc = np.ascontiguousarray(acc, np.int32).copy()
+        rs = np.ascontiguousarray(row_set, np.int32)
+        raw_hi = np.zeros(acc.shape, np.float64)
+        _ck(lib().tfhe_gpu_extern_product_split_hi_batch(self.ptr, i32p(acc), i32p(rs), f64p(raw_hi), acc.shape[0]),
+            "extern_product_split_hi")
+        return acc, raw_hi
+
+    @property
+    def split_bits(self):
+        b = ctypes.c_int32()
+        _ck(lib().tfhe_gpu_key_split_bits(self.ptr, ctypes.byref(b)), "key_split_bits")
+        return b.value
 
     def keyswitch(self, u):
         u = np.ascontiguousarray(u, np.int32)

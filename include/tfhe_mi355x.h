@@ -157,7 +157,7 @@ typedef struct TFheGateBootstrappingCloudKeySet {   /* tb :240-244, :268, :293-2
     const LweBootstrappingKeyFFT* bkFFT;
 } TFheGateBootstrappingCloudKeySet;
 
-typedef struct TFheGateBootstrappingSecretKeySet {  /* alice.c:15, verify.c:147-151 */
+typedef struct TFheGateBootstrappingSecretKeySet {  /* alice.c:15, verify.c:9-13 */
     const TFheGateBootstrappingParameterSet* params;
     const LweKey* lwe_key;
     const TGswKey* tgsw_key;
@@ -192,7 +192,7 @@ void delete_gate_bootstrapping_parameters(TFheGateBootstrappingParameterSet* par
 /* the N=2048 ring (n=750, l=3, Bgbit=8, t=10, basebit=2, stdev 2^-17 / 2^-30); not an upstream symbol */
 TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters_N2048(void);
 
-/* key sets -- alice.c:15, alice.c:62, cloud.c:137 */
+/* key sets -- alice.c:15, alice.c:62, cloud.c:73 */
 TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secret_keyset(
     const TFheGateBootstrappingParameterSet* params);
 void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSecretKeySet* keyset);
@@ -246,7 +246,7 @@ void bootsMUX(LweSample* result, const LweSample* a, const LweSample* b, const L
               const TFheGateBootstrappingCloudKeySet* bk);
 
 /* bootstrapping internals (GPU) -- tb :280 (tfhe_bootstrap_FFT), tb :325 (tfhe_blindRotate_FFT),
- * kernel/src/tfhe_bootstrap_woKS_FFT/tfhe_bootstrap_woKS_FFT.cpp:240 (woKS),
+ * kernel/src/tfhe_bootstrap_woKS_FFT/tfhe_bootstrap_woKS_FFT.cpp:27 (woKS),
  * kernel/src/lwe-keyswitch-functions.cpp:228 (lweKeySwitch). */
 void tfhe_bootstrap_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
 void tfhe_bootstrap_woKS_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
@@ -341,6 +341,8 @@ int tfhe_gpu_key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 /* split point of the SPLIT32 key: key = hi * 2^split + lo.  Applies to keys created afterwards;
  * 0 restores the default, 24 for N = 1024 and 25 for N = 2048. */
 int tfhe_gpu_set_split_bits(int32_t bits);
+/* split point of a device key's SPLIT32 transform (fixed when the key was created) */
+int tfhe_gpu_key_split_bits(const tfhe_gpu_key* key, int32_t* bits);
 /* precision used by the upstream-API entry points (boots*, tfhe_bootstrap_FFT, ...) */
 int tfhe_gpu_set_default_precision(int32_t prec);
 /* launch form of the FP64 bootstrap (no reference counterpart: a tuning knob of this engine).
@@ -395,6 +397,14 @@ int tfhe_gpu_extract_batch(const tfhe_flat_params* p, int32_t* u, const int32_t*
  * pre-rounding values in integer units [count][k+1][N] (FP64). */
 int tfhe_gpu_extern_product_batch(tfhe_gpu_key* key, int32_t prec, int32_t* acc, const int32_t* row_set,
                                   double* raw, int32_t count);
+/* The split-FP32 external product (as tfhe_gpu_extern_product_batch with TFHE_PREC_SPLIT32) that
+ * also returns, in raw_hi [count][k+1][N], the product of the digits with the hi key halves before
+ * rounding, in units of 2^split (tfhe_gpu_key_split_bits).  That product is an exact integer, and
+ * the split path is exact only while its FP32 value stays within 1/2 of it; the distance measures
+ * the margin.  Replaces nothing in the reference: it instruments the split form of the rounding at
+ * TorusPolynomial_fft.cpp:39-41. */
+int tfhe_gpu_extern_product_split_hi_batch(tfhe_gpu_key* key, int32_t* acc, const int32_t* row_set, double* raw_hi,
+                                           int32_t count);
 /* (X^bara - 1) * acc and its gadget decomposition, as computed inside the bootstrap kernel. */
 int tfhe_gpu_rotate_decompose_batch(const tfhe_flat_params* p, int32_t* rot, int32_t* digits,
                                     const int32_t* acc, const int32_t* bara, int32_t count);

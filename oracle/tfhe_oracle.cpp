@@ -547,8 +547,8 @@ extern "C" void orc_extern_product_fft(const orc_params* p, int32_t* acc, const 
 /* blind rotation, extraction, keyswitch, bootstrapping                     */
 /* ------------------------------------------------------------------------ */
 
-/* tfhe_blindRotate_FFT.cpp:179-195 with the loop bound n (the device code stops at 1, :179),
- * loop body = tfhe_MuxRotate_FFT.cpp:101-130: t = (X^a - 1) acc; t = bk_i [.] t; acc = t + acc.
+/* tfhe_blindRotate_FFT.cpp:112-127 with the loop bound n (the device code stops at 1, :112),
+ * loop body = tfhe_MuxRotate_FFT.cpp:56-85: t = (X^a - 1) acc; t = bk_i [.] t; acc = t + acc.
  * Uses the exact int64 external product when fkey == NULL, else the FP64 FFT path. */
 static void blind_rotate(const orc_params* p, int32_t* acc, const int32_t* bara, const int32_t* bk,
                          const FftKey* fk, Scratch& s) {
@@ -557,13 +557,13 @@ static void blind_rotate(const orc_params* p, int32_t* acc, const int32_t* bara,
     const size_t per = (size_t)(k + 1) * p->l * (k + 1) * (N / 2);
     for (int i = 0; i < p->n; ++i) {
         const int32_t a = bara[i];
-        if (a == 0) continue;                                         /* :181 */
+        if (a == 0) continue;                                         /* :114 */
         for (int q = 0; q <= k; ++q)
             orc_poly_mul_xai_minus_one(s.tmp.data() + (size_t)q * N, a, acc + (size_t)q * N, N);
         if (fk) extern_product_fft(p, s.tmp.data(), fk->data.data() + i * per, nullptr, 0, s.dec, s.D, s.S, s.out);
         else orc_extern_product_exact(p, s.tmp.data(), bk + i * tgsw_sz, nullptr);
         for (size_t j = 0; j < (size_t)(k + 1) * N; ++j)
-            acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)s.tmp[j]);  /* tLweAddTo, :93-99 */
+            acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)s.tmp[j]);  /* tLweAddTo, tfhe_MuxRotate_FFT.cpp:48-54 */
     }
 }
 
@@ -573,7 +573,7 @@ extern "C" void orc_blind_rotate(const orc_params* p, int32_t* acc, const int32_
     blind_rotate(p, acc, bara, bk, (const FftKey*)fkey, s);
 }
 
-/* tfhe_bootstrap_woKS_FFT.cpp:276-279 (testvect = [mu]*N), mulByXaiOrCopy.cpp:310-326 (X^{2N-barb}),
+/* tfhe_bootstrap_woKS_FFT.cpp:63-66 (testvect = [mu]*N), mulByXaiOrCopy.cpp:25-41 (X^{2N-barb}),
  * tb :317 (tLweNoiselessTrivial: a = 0, b = v). */
 extern "C" void orc_test_vector(const orc_params* p, int32_t* acc, int32_t barb, int32_t mu) {
     const int N = p->N, k = p->k;
@@ -628,8 +628,8 @@ static void bootstrap_woks(const orc_params* p, int32_t* u, const int32_t* x, in
     const int n = p->n, N = p->N, k = p->k;
     const int Nx2 = 2 * N;
     std::vector<int32_t> bara(n);
-    int32_t barb = orc_modswitch_from_torus32(x[n], Nx2);            /* woKS :270 */
-    for (int i = 0; i < n; ++i) bara[i] = orc_modswitch_from_torus32(x[i], Nx2);  /* :271-274 */
+    int32_t barb = orc_modswitch_from_torus32(x[n], Nx2);            /* woKS :57 */
+    for (int i = 0; i < n; ++i) bara[i] = orc_modswitch_from_torus32(x[i], Nx2);  /* :58-61 */
     std::vector<int32_t> acc((size_t)(k + 1) * N);
     orc_test_vector(p, acc.data(), barb, mu);
     blind_rotate(p, acc.data(), bara.data(), bk, fk, s);

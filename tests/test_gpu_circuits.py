@@ -219,3 +219,76 @@ def test_circuit_level_entry_point_matches_gate_batch(gpu_env):
     finally:
         for b in bufs + outs:
             b.free()
+
+
+def test_graph_rebinds_after_key_replaced():
+    """A circuit bound (and graph-captured) with one device key, then run with GRAPH on a key
+    created after the first was freed -- possibly at the same host address -- must rebind and
+    recapture: the outputs decrypt under the new key set."""
+    r = adder(8)
+    B = 8
+    n1 = P.n + 1
+    rng = np.random.default_rng(70)
+    d_in = T().DeviceBuffer(16 * B * n1 * 4)
+    d_out = T().DeviceBuffer(9 * B * n1 * 4)
+    try:
+        for seed in (101, 202):
+            ks = T().KeySet(1024, seed=(seed,))
+            dk = T().DeviceKey.from_keyset(ks)
+            try:
+                x, y = rng.integers(0, 256, B), rng.integers(0, 256, B)
+                ct = encrypt_inputs(ks, np.concatenate([operand_bits(x, 8), operand_bits(y, 8)]))
+                T().lib().tfhe_gpu_memcpy_h2d(d_in.ptr, ct.ctypes.data_as(T()._vp), ct.nbytes)
+                print(f"[circuits] graph run on a fresh key (seed {seed})", flush=True)
+                r.c.run_dev(dk, d_in, d_out, B, prec=0, graph=True)
+                T().sync()
+                got = decrypt_outputs(ks, d_out.to_array((9, B, n1)))
+                assert from_bits(got) == [int(a + b) for a, b in zip(x, y)], seed
+            finally:
+                dk.free()
+                ks.free()
+    finally:
+        d_in.free()
+        d_out.free()
+        r.c.free()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_adders_4096_full_workload(gpu_env, prec):
+    """The 4096-adder workload itself: 4096 independent 32-bit ripple-carry adders evaluated as one
+    circuit over a batch of 4096 (33 bootstrapped levels: 32 x 4096 XOR, then MUX + XOR levels of
+    12,288 bootstraps), graph-replayed with autotuned launch forms.  Every sum and carry matches
+    plain addition (edge operands included); on FP64 the 33 outputs of the first and the last
+    instance are bit-exact against the oracle evaluating the same netlist gate by gate (the
+    reference's circuit shape: gates called in host loops, test_bench/server/cloud.c:8-37)."""
+    ks, dk, keys = gpu_env
+    B = 4096
+    n1 = P.n + 1
+    rng = np.random.default_rng(3 + prec)
+    x = rng.integers(0, 2**32, B, dtype=np.uint64)
+    y = rng.integers(0, 2**32, B, dtype=np.uint64)
+    x[:3] = [0, 2**32 - 1, 2**31]
+    y[:3] = [0, 1, 2**31]
+    x[-1], y[-1] = 2**32 - 1, 2**32 - 1
+    r = adder(32)
+    bits = np.concatenate([operand_bits(x, 32), operand_bits(y, 32)])
+    ct = encrypt_inputs(ks, bits)
+    d_in = T().DeviceBuffer.from_array(ct)
+    d_out = T().DeviceBuffer(33 * B * n1 * 4)
+    try:
+        print(f"[circuits] 4096 32-bit adders prec={prec}", flush=True)
+        r.c.run_dev(dk, d_in, d_out, B, prec=prec, graph=True, autotune=True)
+        T().sync()
+        out = d_out.to_array((33, B, n1))
+        dec = np.stack([T().decrypt_dev(d_out.ptr.value + i * B * n1 * 4, ks.lwe_key, B) for i in range(33)])
+        assert from_bits(dec[:32]) == [int((a + b) % 2**32) for a, b in zip(x, y)]
+        assert list(dec[32]) == [int((a + b) >> 32) for a, b in zip(x, y)]
+        assert np.array_equal(decrypt_outputs(ks, out[:, [0, B - 1]]), dec[:, [0, B - 1]])
+        if prec == 0:
+            for g in (0, B - 1):
+                print(f"[circuits] oracle gate-by-gate evaluation of instance {g}", flush=True)
+                assert np.array_equal(out[:, g], oracle_eval(keys, r, ct[:, g])), g
+    finally:
+        d_in.free()
+        d_out.free()
+        r.c.free()

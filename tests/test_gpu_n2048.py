@@ -327,3 +327,63 @@ def test_split_point_override_n2048(env2048):
         assert np.array_equal(ks.decrypt(out), 1 - (bits[0] & bits[1]))
     finally:
         dk26.free()
+
+
+def test_split_hi_rounding_margin_n2048(env2048):
+    """As at N=1024: the hi-half product before rounding stays within 0.4 of its exact integer over
+    131,072 coefficients (32 external products), 8 of them against the exact int64 product with the
+    hi-half key.  The default split point at N=2048 is 25 bits (hi half 7 bits)."""
+    ks, dk, keys = env2048
+    assert dk.split_bits == 25
+    rng = np.random.default_rng(2425)
+    cnt = 32
+    acc = rand_torus(rng, (cnt, 2, N))
+    rows = rng.integers(0, P.n, cnt).astype(np.int32)
+    got, hi = dk.extern_product_split_hi(acc, rows)
+    assert np.array_equal(got, dk.extern_product(acc, rows, prec=1))
+    dist = np.abs(hi - np.rint(hi))
+    worst_exact = 0.0
+    for g in range(8):
+        hk = ((keys.bk[rows[g]].astype(np.int64) + (1 << 24)) >> 25).astype(np.int32)
+        _, exact = O.extern_product_exact(P, acc[g], hk, want_raw=True)
+        worst_exact = max(worst_exact, float(np.abs(hi[g] - exact.astype(np.float64)).max()))
+    print(f"N=2048 split 25: hi product distance to an integer max {dist.max():.4f} rms {np.sqrt((dist**2).mean()):.4f}"
+          f" over {dist.size} coefficients; max |hi - exact| = {worst_exact:.4f}; |hi| max {np.abs(hi).max():.3e}",
+          flush=True)
+    assert dist.max() < 0.4
+    assert worst_exact < 0.4
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mux_xor_8192_full_workload(env2048, prec):
+    """The N=2048 workload itself: 8192 MUX (16,384 bootstraps in one launch) and 8192 XOR on random
+    bits through the device-resident gate entry point.  Every output decrypts to its plaintext gate;
+    on FP64 the four gates at each end of both batches (MUX: both segment ends of its bootstrap
+    launch) are bit-exact against the oracle."""
+    ks, dk, keys = env2048
+    G = 8192
+    n1 = P.n + 1
+    rng = np.random.default_rng(5 + prec)
+    bits = rng.integers(0, 2, (3, G)).astype(np.int32)
+    cts = [ks.encrypt(bits[i]) for i in range(3)]
+    bufs = [T().DeviceBuffer.from_array(c) for c in cts]
+    outs = [T().DeviceBuffer(G * n1 * 4) for _ in range(2)]
+    try:
+        print(f"[n2048] 8192 MUX + 8192 XOR prec={prec}", flush=True)
+        dk.gate_dev("MUX", outs[0].ptr, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, G, prec=prec)
+        dk.gate_dev("XOR", outs[1].ptr, bufs[0].ptr, bufs[1].ptr, None, G, prec=prec)
+        T().sync()
+        mux = outs[0].to_array((G, n1))
+        xor = outs[1].to_array((G, n1))
+        assert np.array_equal(ks.decrypt(mux), np.where(bits[0] == 1, bits[1], bits[2]))
+        assert np.array_equal(ks.decrypt(xor), bits[0] ^ bits[1])
+        if prec == 0:
+            idx = [0, 1, 2, 3, G - 4, G - 3, G - 2, G - 1]
+            sel = [c[idx] for c in cts]
+            want_mux, _ = O.gate_batch(keys, "MUX", sel[0], sel[1], sel[2], nthreads=8)
+            want_xor, _ = O.gate_batch(keys, "XOR", sel[0], sel[1], nthreads=8)
+            assert np.array_equal(mux[idx], want_mux)
+            assert np.array_equal(xor[idx], want_xor)
+    finally:
+        for b in bufs + outs:
+            b.free()

@@ -193,3 +193,36 @@ def test_bootstrap_woks_fp64_bit_exact(gpu_env, rng):
     for g in range(5):
         assert np.array_equal(u[g], O.bootstrap_woks(keys, x[g], T().MU)), g
         assert np.array_equal(u[g], O.extract(P, acc[g])), g
+
+
+def hi_half(bk, bits):
+    """The hi halves the device splits each key coefficient into: K = hi * 2^bits + lo with
+    lo in [-2^(bits-1), 2^(bits-1))."""
+    return ((bk.astype(np.int64) + (1 << (bits - 1))) >> bits).astype(np.int32)
+
+
+def test_split_hi_rounding_margin(gpu_env):
+    """The split path is exact only while the product of the digits with the hi key halves rounds to
+    its exact integer value (a slip costs 2^24 units at once).  Over 131,072 coefficients (64
+    external products) the FP32 value before rounding stays within 0.25 of an integer; on 16 of
+    them it is compared with the exact int64 product of the same digits and hi halves (the oracle's
+    exact external product with the hi-half key); the rounded product is the one the plain split
+    entry point returns; and the default split point at N=1024 is 24 bits."""
+    _, dk, keys = gpu_env
+    assert dk.split_bits == 24
+    rng = np.random.default_rng(424)
+    cnt = 64
+    acc = rand_torus(rng, (cnt, 2, N))
+    rows = rng.integers(0, P.n, cnt).astype(np.int32)
+    got, hi = dk.extern_product_split_hi(acc, rows)
+    assert np.array_equal(got, dk.extern_product(acc, rows, prec=1))
+    dist = np.abs(hi - np.rint(hi))
+    worst_exact = 0.0
+    for g in range(16):
+        _, exact = O.extern_product_exact(P, acc[g], hi_half(keys.bk[rows[g]], 24), want_raw=True)
+        worst_exact = max(worst_exact, float(np.abs(hi[g] - exact.astype(np.float64)).max()))
+    print(f"N=1024 split 24: hi product distance to an integer max {dist.max():.4f} rms {np.sqrt((dist**2).mean()):.4f}"
+          f" over {dist.size} coefficients; max |hi - exact| = {worst_exact:.4f}; |hi| max {np.abs(hi).max():.3e}",
+          flush=True)
+    assert dist.max() < 0.25
+    assert worst_exact < 0.25

@@ -4,6 +4,7 @@ reproduce every key array and ciphertext exactly; malformed files must be refuse
 of the demo flow (tests/c/demo_flow.c) must write files the library reads back."""
 import ctypes
 import os
+import struct
 import subprocess
 
 import numpy as np
@@ -161,6 +162,42 @@ def test_malformed_files_are_refused(keyset, tmp_path):
     assert not L.new_tfheGateBootstrappingCloudKeySet_fromFile(f)
     libc.fclose(f)
 
+
+
+def params_header(n, N, k, l, Bgbit, ks_t, ks_basebit):
+    """A TMPS object with the given sizes (the layout tfhe_io.cpp documents)."""
+    return (b"TMPS" + struct.pack("<I", 1) + struct.pack("<7i", n, N, k, l, Bgbit, ks_t, ks_basebit)
+            + struct.pack("<4d", 2.0**-15, 2.0**-15, 2.0**-25, 2.0**-25))
+
+
+@pytest.mark.parametrize("sizes", [
+    (630, 1024, 1, 3, 7, 1, 31),          # keyswitch base 2^31: must be refused before it is formed
+    (630, 1024, 1, 3, 7, 1, 32),
+    (4096, 65536, 4, 16, 2, 8, 2),        # within the per-field limits, but a ~430 GB bootstrapping key
+    (4096, 65536, 1, 3, 7, 16, 2),        # a ~69 GB keyswitch key
+])
+def test_crafted_headers_are_refused(tmp_path, sizes):
+    """Parameter headers whose keys could not be allocated are refused with NULL and an error
+    message, for each reader, before any key storage is allocated."""
+    L = T.lib()
+    hdr = params_header(*sizes)
+    files = [(L.new_tfheGateBootstrappingParameterSet_fromFile, hdr),
+             (L.new_tfheGateBootstrappingCloudKeySet_fromFile, b"TMCK" + struct.pack("<I", 1) + hdr),
+             (L.new_tfheGateBootstrappingSecretKeySet_fromFile, b"TMSK" + struct.pack("<I", 1) + hdr)]
+    for reader, data in files:
+        (tmp_path / "crafted").write_bytes(data + b"\0" * 256)
+        f = fopen(tmp_path / "crafted", "rb")
+        assert not reader(f), sizes
+        libc.fclose(f)
+        assert L.tfhe_gpu_last_error()
+    # the supported sets still read back
+    for ok in ((630, 1024, 1, 3, 7, 8, 2), (750, 2048, 1, 3, 8, 10, 2)):
+        (tmp_path / "ok").write_bytes(params_header(*ok))
+        f = fopen(tmp_path / "ok", "rb")
+        p = L.new_tfheGateBootstrappingParameterSet_fromFile(f)
+        libc.fclose(f)
+        assert p
+        L.delete_gate_bootstrapping_parameters(p)
 
 def build_demo(tmp_path):
     exe = tmp_path / "demo_flow"
