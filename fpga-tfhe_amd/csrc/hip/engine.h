@@ -85,6 +85,14 @@ struct DevKeyView {
     int ksk_np = 0;                  // padded output length (multiple of 64)
 };
 
+// Fixed device scratch of the no-allocation gate entry point: kScratchElems int32 (16 MiB), one
+// copy per device for the life of the process.  scratch_init resolves its address on the current
+// device (call once per device at set-up); scratch_ptr returns it (null before scratch_init).
+constexpr size_t kScratchElems = (size_t)4 << 20;
+constexpr int kScratchMaxDevices = 64;
+hipError_t scratch_init(int device);
+int32_t* scratch_ptr(int device);
+
 hipError_t build_twiddles(DevTwiddles& tw, int N);
 void free_twiddles(DevTwiddles& tw);
 

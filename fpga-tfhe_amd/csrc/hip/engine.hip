@@ -2103,6 +2103,29 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fixed scratch for the entry point that allocates nothing (tfhe_gpu_gate_batch_dev_noalloc): the
+// extracted samples between the bootstrap and the keyswitch live in this array instead of a buffer
+// grown with hipMalloc.  Host code never uses its name as an address: scratch_init() resolves the
+// device address once per device (with that device current) and every launch takes that pointer.
+// ---------------------------------------------------------------------------------------------
+static __device__ __align__(256) int32_t scratch[kScratchElems];
+static void* g_scratch_ptr[kScratchMaxDevices] = {};
+
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return hipErrorInvalidDevice;
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+int32_t* scratch_ptr(int device) {
+    return (device >= 0 && device < kScratchMaxDevices) ? (int32_t*)g_scratch_ptr[device] : nullptr;
+}
+
 // FP64 bootstrap form when a launch does not name one: 0 one bootstrap per workgroup, 1 workgroups
 // sharing key rows, 2 (default) by launch size (boot64_single_round, boot64_latency_max)
 static int g_boot64_shared = 2;

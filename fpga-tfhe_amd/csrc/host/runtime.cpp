@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -105,6 +106,8 @@ int ensure_init() {
     }
     HIPR(hipSetDevice(c.device));
     HIPR(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    // device address of the fixed scratch of tfhe_gpu_gate_batch_dev_noalloc, with the device current
+    HIPR(scratch_init(c.device));
     c.inited = true;
     return TFHE_OK;
 }
@@ -216,9 +219,12 @@ int check_key_prec(tfhe_gpu_key* key, int32_t prec) {
 
 const char* boot_name(int32_t prec) { return prec == TFHE_PREC_FP64 ? "bootstrap_fp64" : "bootstrap_split32"; }
 
-// device-resident gate batch; all pointers are device pointers
+// device-resident gate batch; all pointers are device pointers.  The extracted samples go to the
+// key's scratch (grown with hipMalloc on demand), or, with fixed_u (the no-allocation entry point),
+// to that buffer of fixed_elems int32: then nothing is allocated and no profiling events are made.
 int gate_batch_dev_impl(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* out, const int32_t* a,
-                        const int32_t* b, const int32_t* c, int32_t count) {
+                        const int32_t* b, const int32_t* c, int32_t count, int32_t* fixed_u = nullptr,
+                        size_t fixed_elems = 0) {
     Ctx& cx = ctx();
     if (count < 0) return arg_fail("negative count");
     if (count == 0) return TFHE_OK;
@@ -227,16 +233,39 @@ int gate_batch_dev_impl(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* 
     int rc = check_key_prec(key, prec);
     if (rc) return rc;
     DevTwiddles* tw;
-    if ((rc = ensure_twiddles(key->p.N, &tw))) return rc;
+    if (!fixed_u) {
+        if ((rc = ensure_twiddles(key->p.N, &tw))) return rc;
+    } else {
+        // built when the key was created; looked up only, never built here
+        auto it = cx.tw.find(key->p.N);
+        if (it == cx.tw.end()) return arg_fail("no transform tables for this ring size (create the key first)");
+        tw = &it->second;
+    }
     const int N = key->p.N;
     const int32_t MU = (int32_t)(1u << 29);
     const int nboot = (gate == TFHE_GATE_MUX) ? 2 * count : count;
-    if ((rc = key_u_buffer(key, (size_t)nboot * (N + 1)))) return rc;
+    const size_t u_elems = (size_t)nboot * (N + 1);
+    int32_t* u;
+    if (fixed_u) {
+        if (u_elems > fixed_elems) {
+            char msg[256];
+            snprintf(msg, sizeof msg,
+                     "tfhe_gpu_gate_batch_dev_noalloc: %d gates need %zu int32 of scratch, the fixed scratch holds "
+                     "%zu (at most %zu bootstraps at N=%d; a MUX is two)",
+                     count, u_elems, fixed_elems, fixed_elems / (N + 1), N);
+            return arg_fail(msg);
+        }
+        u = fixed_u;
+    } else {
+        if ((rc = key_u_buffer(key, u_elems))) return rc;
+        u = key->d_u;
+    }
+    const bool prof = fixed_u == nullptr;
     BootLaunch L;
     L.mode = 0;
     L.count = nboot;
     L.mu = MU;
-    L.u_out = key->d_u;
+    L.u_out = u;
     if (gate != TFHE_GATE_MUX) {
         L.nseg = 1;
         L.seg_start[1] = count;
@@ -254,16 +283,15 @@ int gate_batch_dev_impl(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* 
         gate_coeffs(TFHE_GATE_ANDNY, &L.cst[1], &L.alpha[1], &L.beta[1]);
     }
     {
-        ProfScope ps(cx, boot_name(prec));
+        std::unique_ptr<ProfScope> ps(prof ? new ProfScope(cx, boot_name(prec)) : nullptr);
         hipError_t e = launch_bootstrap(key->view(), *tw, (Prec)prec, L, cx.stream);
         if (e != hipSuccess) return hip_fail("bootstrap launch", e);
     }
     {
-        ProfScope ps(cx, "keyswitch");
+        std::unique_ptr<ProfScope> ps(prof ? new ProfScope(cx, "keyswitch") : nullptr);
         hipError_t e = (gate == TFHE_GATE_MUX)
-                           ? launch_keyswitch(key->view(), out, key->d_u, key->d_u + (size_t)count * (N + 1), MU,
-                                              count, cx.stream)
-                           : launch_keyswitch(key->view(), out, key->d_u, nullptr, 0, count, cx.stream);
+                           ? launch_keyswitch(key->view(), out, u, u + (size_t)count * (N + 1), MU, count, cx.stream)
+                           : launch_keyswitch(key->view(), out, u, nullptr, 0, count, cx.stream);
         if (e != hipSuccess) return hip_fail("keyswitch launch", e);
     }
     return TFHE_OK;
@@ -557,6 +585,26 @@ extern "C" int tfhe_gpu_gate_batch_dev(tfhe_gpu_key* key, int32_t prec, int32_t 
     int rc = ensure_init();
     if (rc) return rc;
     return gate_batch_dev_impl(key, prec, gate, d_out, d_a, d_b, d_c, count);
+}
+
+// The same without allocating: the extracted samples use the library's fixed device scratch (its
+// address resolved at init), so no hipMalloc, hipFree or event creation happens here.  Calls that
+// overlap on one device share that scratch and race; calls on the library stream are serialised.
+extern "C" int tfhe_gpu_gate_batch_dev_noalloc(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* d_out,
+                                               const int32_t* d_a, const int32_t* d_b, const int32_t* d_c,
+                                               int32_t count) {
+    Ctx& c = ctx();
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    if (!c.inited) {
+        g_last_error = "tfhe_gpu_gate_batch_dev_noalloc: call tfhe_gpu_init (or create the key) first";
+        return TFHE_E_NODEV;
+    }
+    int32_t* u = scratch_ptr(c.device);
+    if (!u) {
+        g_last_error = "tfhe_gpu_gate_batch_dev_noalloc: fixed scratch not resolved on this device";
+        return TFHE_E_NODEV;
+    }
+    return gate_batch_dev_impl(key, prec, gate, d_out, d_a, d_b, d_c, count, u, kScratchElems);
 }
 
 extern "C" int tfhe_gpu_gate_batch(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* out, const int32_t* a,

@@ -104,6 +104,7 @@ def lib():
         "tfhe_gpu_key_params": (ctypes.c_int, [_vp, P(FlatParams)]),
         "tfhe_gpu_gate_batch": (ctypes.c_int, [_vp, i32, i32, _i32p, _i32p, _i32p, _i32p, i32]),
         "tfhe_gpu_gate_batch_dev": (ctypes.c_int, [_vp, i32, i32, _vp, _vp, _vp, _vp, i32]),
+        "tfhe_gpu_gate_batch_dev_noalloc": (ctypes.c_int, [_vp, i32, i32, _vp, _vp, _vp, _vp, i32]),
         "tfhe_gpu_boots_batch": (ctypes.c_int, [i32, P(LweSample), P(LweSample), P(LweSample),
                                                 P(LweSample), i32, _vp]),
         "tfhe_gpu_bootstrap_woks_batch": (ctypes.c_int, [_vp, i32, _i32p, _i32p, i32, i32]),

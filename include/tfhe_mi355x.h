@@ -377,6 +377,16 @@ int tfhe_gpu_gate_batch(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* 
 /* The same on device-resident arrays; stream may be NULL (library stream). */
 int tfhe_gpu_gate_batch_dev(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* d_out,
                             const int32_t* d_a, const int32_t* d_b, const int32_t* d_c, int32_t count);
+/* tfhe_gpu_gate_batch_dev without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The extracted samples between bootstrap and keyswitch use the
+ * library's fixed 16 MiB device scratch (resolved by tfhe_gpu_init) instead of a buffer grown on
+ * demand: at most 4,194,304 / (N+1) bootstraps per call (4092 gates at N = 1024, 2047 at N = 2048;
+ * a MUX is two bootstraps); a larger count returns TFHE_E_ARG and launches nothing.  The device
+ * must have been initialised (tfhe_gpu_init or a key created) beforehand.  Calls that overlap on
+ * one device share the scratch and race; calls on the library stream are serialised. */
+int tfhe_gpu_gate_batch_dev_noalloc(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* d_out,
+                                    const int32_t* d_a, const int32_t* d_b, const int32_t* d_c, int32_t count);
 /* Batched gates over upstream LweSample arrays (same gate codes). */
 int tfhe_gpu_boots_batch(int32_t gate, LweSample* out, const LweSample* a, const LweSample* b,
                          const LweSample* c, int32_t count, const TFheGateBootstrappingCloudKeySet* bk);
