@@ -28,6 +28,8 @@ M = N // 2
 # algorithmic work per gate: n * [8 transforms * (5 M log2 M + 6 M) + (k+1) l (k+1) M complex MACs * 8]
 FLOP_PER_GATE = n * (8 * (5 * M * 9 + 6 * M) + (k + 1) * l * (k + 1) * M * 8)   # 162,570,240
 PEAK_TFLOPS = {"fp64": 78.6, "split32": 157.3}  # MI355X published vector/matrix peaks
+FORM_NAMES = {0: "size rule", 1: "key-sharing workgroups (8 bootstraps)", 2: "one bootstrap per workgroup",
+              3: "latency form", 4: "2 bootstraps per workgroup", 5: "4 bootstraps per workgroup"}
 
 
 def parse():
@@ -45,6 +47,8 @@ def parse():
     ap.add_argument("--adders", type=int, default=4096, help="32-bit adders per circuit run (0: skip)")
     ap.add_argument("--n2048-gates", type=int, default=8192,
                     help="MUX and XOR gates each at the N=2048 set (0: skip)")
+    ap.add_argument("--split-variants", action="store_true",
+                    help="also measure the split-FP32 forms (24/16-bit split x plain/compensated accumulation)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -146,6 +150,85 @@ def hi_margin(dk, acc, rows):
             "samples": int(d.size), "slip_at": 0.5}
 
 
+def negacyclic_mul_binary(a, s):
+    """a * s mod (X^N + 1) for an int64 polynomial a and a binary key s (exact in int64)."""
+    Nn = a.shape[-1]
+    c = np.convolve(a.astype(np.int64), s.astype(np.int64))
+    out = c[:Nn].copy()
+    out[:Nn - 1] -= c[Nn:]
+    return out
+
+
+def split_variants_report(T, ks, batch, steps, audit_gates, rng):
+    """The split-FP32 external product in four forms: key halves split at 24 bits (hi 8 bits, the
+    default) or at 16 bits (two 16-bit halves), each with plain FP32 FMA accumulation or with
+    compensated accumulation (TwoProduct + TwoSum).  For each: gates/s on the NAND batch, the
+    external-product error against the FP64 path (the exact int64 product; 64 products), the TLWE
+    phase error after a full blind rotation (8 accumulators of a launch of 300, against the
+    noiseless rotation), and the decryption failures of the chained NAND audit."""
+    nn = ks.fp.n
+    bits = rng.integers(0, 2, (2, batch)).astype(np.int32)
+    d_a = T.DeviceBuffer.from_array(ks.encrypt(bits[0]))
+    d_b = T.DeviceBuffer.from_array(ks.encrypt(bits[1]))
+    d_out = T.DeviceBuffer(batch * (nn + 1) * 4)
+    want = 1 - (bits[0] & bits[1])
+    acc = rng.integers(-2**31, 2**31, (64, 2, N), dtype=np.int64).astype(np.int32)
+    rows = rng.integers(0, nn, 64).astype(np.int32)
+    cnt = 300
+    acc0 = np.zeros((cnt, 2, N), np.int32)
+    acc0[:, 1, :] = 1 << 29
+    bara = rng.integers(0, 2 * N, (cnt, nn)).astype(np.int32)
+    skey = ks.tlwe_key[:N].astype(np.int64)
+    out = {}
+    try:
+        ref64 = None
+        for bits_split, mac in ((24, 0), (24, 1), (16, 0), (16, 1)):
+            name = f"split{bits_split}_{'compensated' if mac else 'plain'}"
+            T.set_split_bits(bits_split)
+            try:
+                dk = T.DeviceKey.from_keyset(ks)
+            finally:
+                T.set_split_bits(0)
+            T.set_split_mac(mac)
+            try:
+                if ref64 is None:
+                    ref64 = dk.extern_product(acc, rows, prec=T.PREC_FP64)
+                r = time_path(T, dk, T.PREC_SPLIT32, (d_out, d_a, d_b), batch, steps, 1)
+                r["correct"] = bool(np.array_equal(T.decrypt_dev(d_out, ks.lwe_key, batch), want))
+                r32 = dk.extern_product(acc, rows, prec=T.PREC_SPLIT32)
+                err = np.abs(((r32.astype(np.int64) - ref64.astype(np.int64) + 2**31) % 2**32) - 2**31).ravel()
+                hist = np.bincount(np.ceil(np.log2(err.astype(np.float64) + 1)).astype(int), minlength=1)
+                rot = dk.blind_rotate(acc0, bara, prec=T.PREC_SPLIT32)
+                perr = []
+                for g in range(8):
+                    ph = (rot[g, 1].astype(np.int64) - negacyclic_mul_binary(rot[g, 0], skey))
+                    m = int((bara[g].astype(np.int64) * ks.lwe_key).sum() % (2 * N))
+                    ideal = np.where((np.arange(N) - m) % (2 * N) < N, 1, -1).astype(np.int64) << 29
+                    perr.append((ph - ideal + 2**31) % 2**32 - 2**31)
+                pe = np.abs(np.concatenate(perr)).astype(np.float64)
+                done, fails = audit(T, dk, ks, T.PREC_SPLIT32, batch, audit_gates, np.random.default_rng(2))
+                out[name] = {"split_bits": bits_split, "accumulation": "compensated" if mac else "plain FMA",
+                             "gates_per_s": round(r["gates_per_s"], 1), "bootstrap_kernel_ms": round(r["boot_ms_avg"], 3),
+                             "correct_batch": r["correct"],
+                             "extern_product_error": {"max": int(err.max()),
+                                                      "rms": round(float(np.sqrt((err.astype(float) ** 2).mean())), 1),
+                                                      "samples": int(err.size),
+                                                      "log2_histogram": {f"<=2^{i}": int(c) for i, c in enumerate(hist) if c}},
+                             "phase_error_after_blind_rotation": {"rms": round(float(np.sqrt((pe ** 2).mean())), 1),
+                                                                  "max": int(pe.max()), "steps": nn,
+                                                                  "samples": int(pe.size), "decision_margin": 2**28},
+                             "audit": {"gates": done, "failures": fails, "chained": True},
+                             "unit": "2^-32 torus"}
+                log(f"[bench] split variant {name}: {out[name]}")
+            finally:
+                T.set_split_mac(0)
+                dk.free()
+    finally:
+        for b_ in (d_a, d_b, d_out):
+            b_.free()
+    return out
+
+
 def adders_report(T, dk, ks, count, reps=2):
     """`count` independent 32-bit ripple-carry adders as one circuit over a batch of `count`
     instances: 33 bootstrapped levels (one of 32 XOR, then MUX + XOR along the carry chain), each
@@ -181,8 +264,12 @@ def adders_report(T, dk, ks, count, reps=2):
             s_ = [int(sum(int(dec[i][j]) << i for i in range(32))) for j in range(count)]
             ok = s_ == [int((a + b) % 2**32) for a, b in zip(x, y)] and \
                 list(dec[32]) == [int((a + b) >> 32) for a, b in zip(x, y)]
+            forms = {}
+            for nb, form in c.level_plan():
+                forms[str(nb * count)] = FORM_NAMES.get(form, str(form))
             res[name] = {"ms_per_run": round(sec * 1e3, 2), "adders_per_s": round(count / sec, 1),
-                         "bootstraps_per_s": round(count * info["bootstraps"] / sec, 1), "correct": bool(ok)}
+                         "bootstraps_per_s": round(count * info["bootstraps"] / sec, 1), "correct": bool(ok),
+                         "autotuned_form_by_level_bootstraps": forms}
             log(f"[bench] adders {name}: {res[name]}")
     finally:
         d_in.free()
@@ -405,6 +492,8 @@ def main():
     prec_rep = precision_report(T, dk, np.random.default_rng(3)) if len(precs) == 2 else None
     log(f"[bench] split32 external-product error {prec_rep}")
 
+    variants = (split_variants_report(T, ks, B, max(2, args.steps // 2), args.audit_gates, np.random.default_rng(4))
+                if args.split_variants else None)
     adders = adders_report(T, dk, ks, args.adders) if args.adders > 0 and args.prec == "both" else None
     n2048 = n2048_report(T, args.n2048_gates) if args.n2048_gates > 0 and args.prec == "both" else None
 
@@ -464,6 +553,7 @@ def main():
                      "measured_valu_peak": {"fp64": round(vp64, 1), "fp32_unpacked": round(vp32, 1)},
                      "key_stream": key_stream},
         "cpu_baseline": cpu,
+        "split32_variants": variants,
         "circuits_32bit_adders": adders,
         "n2048": n2048,
         "device": dev,

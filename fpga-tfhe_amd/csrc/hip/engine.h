@@ -54,8 +54,14 @@ struct LinDesc {
 //   mode 1: acc_io[g] (k+1 = 2 polys) is blind-rotated by bara_in[g][0..n) in place.
 //   mode 2: acc_io[g] <- bk[row_set[g]] (.) acc_io[g] (one external product); raw_out, raw_hi optional.
 //   mode 3: as mode 0, with the gate inputs of job g taken from segs[g / seg_jobs].
-// form (FP64, N=1024): 0 = the process-wide choice, 1 = key-sharing workgroups, 2 = one bootstrap
-// per workgroup, 3 = one bootstrap per workgroup of 2l wavefronts (latency form; modes 0, 1, 3).
+// form: 0 = the process-wide choice (set_forced_form, else the size rules);
+//   1 = FP64 at N=1024: key-sharing workgroups (k_boot64w);
+//   2 = one bootstrap per workgroup: k_boot64 / k_boot32 (N=1024), k_boot2w with one two-wavefront
+//       bootstrap per workgroup (N=2048);
+//   3 = latency form, one bootstrap per workgroup of several wavefronts (modes 0, 1, 3);
+//   4, 5 = N=2048: k_boot2w with 2 or 4 bootstraps per workgroup.
+// A form that does not apply to a launch's path (1 on the split path or at N=2048, 4 and 5 at
+// N=1024) is treated as 0.
 struct BootLaunch {
     int mode = 0;
     int form = 0;
@@ -100,6 +106,10 @@ bool params_supported(const DevParams& p);
 // FP64 launch form when a launch names none: 0 one bootstrap per workgroup, 1 key-sharing
 // workgroups, 2 by launch size
 void set_boot64_shared(int mode);
+// split-FP32 MAC at N = 1024: 0 plain FMA accumulation, 1 compensated (Dot2) accumulation
+void set_split_mac(int mode);
+// launch form for launches that name none (0 restores the size rules)
+void set_forced_form(int form);
 hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& L,
                             hipStream_t st);
 // KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.  Output g goes to out[g], or,

@@ -549,8 +549,10 @@ __device__ __forceinline__ float4 key_load(__amdgpu_buffer_rsrc_t rs, uint32_t l
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)lane_off, (int)row_off, 0));
 }
 
-// kr: the key buffer; row0: byte offset of this step's row set in it
-template <int N, int L, int BG, bool ROT>
+// kr: the key buffer; row0: byte offset of this step's row set in it.  MAC = 0: the products are
+// accumulated with plain FP32 FMAs; MAC = 1: compensated (cmac_comp), with the compensation terms
+// added into each sum before its inverse transform.
+template <int N, int L, int BG, bool ROT, int MAC = 0>
 __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, __amdgpu_buffer_rsrc_t kr,
                                        uint32_t row0, const DitTw<float>& tw, const C2<float>* tws,
                                        const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw,
@@ -559,10 +561,14 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
     static_assert(M == 512, "the split path runs the 512-point decimation-in-time transform");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float> h0[R1], l0[R1], h1[R1], l1[R1];
+    // compensation terms of the MAC = 1 form (unused, and removed by the compiler, when MAC = 0)
+    C2<float> ch0[R1], cl0[R1], ch1[R1], cl1[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
         h0[r] = C2<float>{0.f, 0.f}; l0[r] = C2<float>{0.f, 0.f};
         h1[r] = C2<float>{0.f, 0.f}; l1[r] = C2<float>{0.f, 0.f};
+        ch0[r] = C2<float>{0.f, 0.f}; cl0[r] = C2<float>{0.f, 0.f};
+        ch1[r] = C2<float>{0.f, 0.f}; cl1[r] = C2<float>{0.f, 0.f};
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -584,15 +590,32 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 const float4 k0 = key_load(kr, (uint32_t)lane * sizeof(float4), ro + r * 64 * sizeof(float4));
-                cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
-                cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
+                if constexpr (MAC == 0) {
+                    cmac(h0[r], v[r], C2<float>{k0.x, k0.y});
+                    cmac(l0[r], v[r], C2<float>{k0.z, k0.w});
+                } else {
+                    cmac_comp(h0[r], ch0[r], v[r], C2<float>{k0.x, k0.y});
+                    cmac_comp(l0[r], cl0[r], v[r], C2<float>{k0.z, k0.w});
+                }
             }
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 const float4 k1 = key_load(kr, (uint32_t)lane * sizeof(float4), ro + (M + r * 64) * sizeof(float4));
-                cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
-                cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
+                if constexpr (MAC == 0) {
+                    cmac(h1[r], v[r], C2<float>{k1.x, k1.y});
+                    cmac(l1[r], v[r], C2<float>{k1.z, k1.w});
+                } else {
+                    cmac_comp(h1[r], ch1[r], v[r], C2<float>{k1.x, k1.y});
+                    cmac_comp(l1[r], cl1[r], v[r], C2<float>{k1.z, k1.w});
+                }
             }
+        }
+    }
+    if constexpr (MAC != 0) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            h0[r] = h0[r] + ch0[r]; l0[r] = l0[r] + cl0[r];
+            h1[r] = h1[r] + ch1[r]; l1[r] = l1[r] + cl1[r];
         }
     }
     fft512_dit<-1, false>(h0, tw, (const C2<float>*)nullptr, s_buf, lane);
@@ -637,9 +660,12 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
     wave_sync();
 }
 
-template <int N, int L, int BG>
+// MAC = 1 (compensated accumulation) doubles the accumulator registers and is compiled for one
+// wavefront per SIMD.
+template <int N, int L, int BG, int MAC = 0>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 ? TFHE_BOOT32_WAVES : 1))) k_boot32(BootKArgs A) {
+__attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1,
+                                   N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1))) k_boot32(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
     __shared__ int32_t s_acc[2 * N];
     __shared__ C2<float> s_buf[xbuf_elems<M>()];
@@ -664,15 +690,15 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 ? TFHE_BOOT32_WAVES : 1, N <= 1024 
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
         double* raw_hi = A.L.raw_hi ? A.L.raw_hi + (size_t)g * 2 * N : nullptr;
-        step32<N, L, BG, false>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, untw, lane, sb,
-                                raw, raw_hi);
+        step32<N, L, BG, false, MAC>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, untw, lane,
+                                     sb, raw, raw_hi);
     } else {
         const int n = A.n;
         for (int i = 0; i < n; ++i) {
             const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
             if (a == 0) continue;
-            step32<N, L, BG, true>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb, nullptr,
-                                   nullptr);
+            step32<N, L, BG, true, MAC>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb,
+                                        nullptr, nullptr);
         }
     }
     boot_epilogue<N>(A, g, lane, s_acc);
@@ -2131,6 +2157,14 @@ int32_t* scratch_ptr(int device) {
 static int g_boot64_shared = 2;
 void set_boot64_shared(int mode) { g_boot64_shared = mode; }
 
+// split-FP32 MAC form at N = 1024: 0 plain FMA accumulation (default), 1 compensated
+static int g_split_mac = 0;
+void set_split_mac(int mode) { g_split_mac = mode; }
+
+// launch form applied to launches that name none (0: the size rules); see BootLaunch::form
+static int g_forced_form = 0;
+void set_forced_form(int form) { g_forced_form = form; }
+
 static int device_cus() {
     static int cus = 0;
     if (!cus) {
@@ -2168,14 +2202,16 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.t1 = tw.t1_fold64;
         A.t2 = tw.t2_64;
         if constexpr (N == 1024) {
+            int form = Lc.form ? Lc.form : g_forced_form;
+            if (form > 3) form = 0;   // the N = 2048 workgroup sizes: the size rules here
             if (Lc.mode != 2 && key.p.n <= kMaxLweN &&
-                (Lc.form == 3 || (Lc.form == 0 && g_boot64_shared == 2 && Lc.count <= boot64_latency_max()))) {
+                (form == 3 || (form == 0 && g_boot64_shared == 2 && Lc.count <= boot64_latency_max()))) {
                 hipLaunchKernelGGL((k_boot64lat<N, L, BG>), dim3(Lc.count), dim3(64 * 2 * L), 0, st, A);
                 return hipGetLastError();
             }
-            const bool shared = Lc.form == 1 ||
-                                (Lc.form == 0 && (g_boot64_shared == 1 ||
-                                                  (g_boot64_shared == 2 && Lc.count > boot64_single_round())));
+            const bool shared = form == 1 ||
+                                (form == 0 && (g_boot64_shared == 1 ||
+                                               (g_boot64_shared == 2 && Lc.count > boot64_single_round())));
             if (Lc.mode != 2 && key.p.n <= B64W_MAXN && shared) {
                 const int blocks = (Lc.count + B64W_GATES - 1) / B64W_GATES;
                 hipLaunchKernelGGL((k_boot64w<N, L, BG>), dim3(blocks), dim3(64 * B64W_GATES), 0, st, A);
@@ -2188,7 +2224,16 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
         A.t1 = tw.t1_plain32;   // W_512^{c L}: pass-3 input twiddles of the decimation-in-time transform
         A.t2 = tw.t2dit32;
         if constexpr (N == 1024) {
-            if (Lc.mode != 2 && Lc.count <= boot64_latency_max()) {
+            if (g_split_mac == 1) {
+                // the compensated-accumulation form exists for the one-bootstrap-per-wavefront kernel
+                hipLaunchKernelGGL((k_boot32<N, L, BG, 1>), dim3(Lc.count), dim3(64), 0, st, A);
+                return hipGetLastError();
+            }
+            // forms: 3 the latency form, 2 one bootstrap per wavefront, 0 (and the forms of other
+            // paths) by launch size
+            int form = Lc.form ? Lc.form : g_forced_form;
+            if (form != 2 && form != 3) form = 0;
+            if (Lc.mode != 2 && (form == 3 || (form == 0 && Lc.count <= boot64_latency_max()))) {
                 hipLaunchKernelGGL((k_boot32lat<N, L, BG>), dim3(Lc.count), dim3(64 * 2 * L), 0, st, A);
                 return hipGetLastError();
             }
@@ -2212,11 +2257,13 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
         A.t2w_rw64 = tw.rw64;
         A.t2w_rw32 = tw.rw32;
         // launches of at most one bootstrap per CU take the latency form (FP64: unless a launch form
-        // other than it is forced, as at N = 1024)
+        // other than it is forced, as at N = 1024).  Forms: 3 latency; 2, 4, 5 the batched kernel
+        // with 1, 2, 4 bootstraps per workgroup; 0 by launch size (FP64 4, split 2 per workgroup).
+        const int form = Lc.form ? Lc.form : g_forced_form;
         const bool lat = Lc.mode != 2 && A.n <= kMaxLweN &&
-                         (prec == PREC_FP64 ? (Lc.form == 3 || (Lc.form == 0 && g_boot64_shared == 2 &&
-                                                                Lc.count <= boot64_latency_max()))
-                                            : Lc.count <= boot64_latency_max());
+                         (form == 3 || (form == 0 && (prec == PREC_SPLIT32 || g_boot64_shared == 2) &&
+                                        Lc.count <= boot64_latency_max()));
+        const int gb = form == 2 ? 1 : form == 4 ? 2 : form == 5 ? 4 : (prec == PREC_FP64 ? 4 : 2);
         if (prec == PREC_FP64) {
             A.bk = key.bk64;
             A.t1 = tw.t1w64;
@@ -2225,10 +2272,11 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
                 hipLaunchKernelGGL((k_boot2wlat<3, 8, 0>), dim3(Lc.count), dim3(64 * 6), 0, st, A);
                 return hipGetLastError();
             }
-            // 8 wavefronts: one workgroup per CU at two wavefronts per SIMD (two bootstraps per
-            // workgroup measured 1% slower here)
-            constexpr int GB = 4;
-            hipLaunchKernelGGL((k_boot2w<3, 8, 0, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
+            // default 4 bootstraps (8 wavefronts): one workgroup per CU at two wavefronts per SIMD
+            // (two bootstraps per workgroup measured 1% slower here)
+            if (gb == 1) hipLaunchKernelGGL((k_boot2w<3, 8, 0, 1>), dim3(Lc.count), dim3(128), 0, st, A);
+            else if (gb == 2) hipLaunchKernelGGL((k_boot2w<3, 8, 0, 2>), dim3((Lc.count + 1) / 2), dim3(256), 0, st, A);
+            else hipLaunchKernelGGL((k_boot2w<3, 8, 0, 4>), dim3((Lc.count + 3) / 4), dim3(512), 0, st, A);
         } else {
             A.bk = key.bk32;
             A.t1 = tw.t1w32;
@@ -2238,12 +2286,13 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
                 hipLaunchKernelGGL((k_boot2wlat<3, 8, 1>), dim3(Lc.count), dim3(64 * 8), 0, st, A);
                 return hipGetLastError();
             }
-            // two bootstraps (4 wavefronts) per workgroup, two workgroups per CU at two wavefronts
-            // per SIMD: fewer wavefronts per barrier than with four per workgroup (3.8% faster on
-            // 4096 MUX + 4096 XOR), while each row's reads still meet in L1 (one per workgroup: 14%
-            // slower)
-            constexpr int GB = 2;
-            hipLaunchKernelGGL((k_boot2w<3, 8, 1, GB>), dim3((Lc.count + GB - 1) / GB), dim3(128 * GB), 0, st, A);
+            // default two bootstraps (4 wavefronts) per workgroup, two workgroups per CU at two
+            // wavefronts per SIMD: fewer wavefronts per barrier than with four per workgroup (3.8%
+            // faster on 4096 MUX + 4096 XOR), while each row's reads still meet in L1 (one per
+            // workgroup: 14% slower)
+            if (gb == 1) hipLaunchKernelGGL((k_boot2w<3, 8, 1, 1>), dim3(Lc.count), dim3(128), 0, st, A);
+            else if (gb == 4) hipLaunchKernelGGL((k_boot2w<3, 8, 1, 4>), dim3((Lc.count + 3) / 4), dim3(512), 0, st, A);
+            else hipLaunchKernelGGL((k_boot2w<3, 8, 1, 2>), dim3((Lc.count + 1) / 2), dim3(256), 0, st, A);
         }
         return hipGetLastError();
     }

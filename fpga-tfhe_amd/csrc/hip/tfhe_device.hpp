@@ -66,6 +66,32 @@ __device__ __forceinline__ void cmac(C2<float>& acc, C2<float> a, C2<float> b) {
     acc.y = __builtin_fmaf(a.y, b.x, acc.y);
 }
 
+// s + e = a + b exactly (Knuth's TwoSum: no ordering of |a| and |b| assumed)
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// acc += a * b as a compensated dot-product step (the Dot2 scheme of Ogita, Rump and Oishi, 2005):
+// each real product is split into its rounded value and its exact rounding error (one FMA), the
+// rounded values are added to acc with TwoSum, and every rounding error -- of the products and of
+// the sums -- is gathered in comp.  acc + comp then carries the dot product as if it were computed
+// in twice the working precision.  About 19 operations per real component against 2 FMAs.
+__device__ __forceinline__ void cmac_comp(C2<float>& acc, C2<float>& comp, C2<float> a, C2<float> b) {
+    const float p0 = a.x * b.x, e0 = __builtin_fmaf(a.x, b.x, -p0);
+    const float p1 = -a.y * b.y, e1 = __builtin_fmaf(-a.y, b.y, -p1);
+    const float p2 = a.x * b.y, e2 = __builtin_fmaf(a.x, b.y, -p2);
+    const float p3 = a.y * b.x, e3 = __builtin_fmaf(a.y, b.x, -p3);
+    float t0, t1, t2, t3;
+    two_sum(acc.x, p0, acc.x, t0);
+    two_sum(acc.x, p1, acc.x, t1);
+    two_sum(acc.y, p2, acc.y, t2);
+    two_sum(acc.y, p3, acc.y, t3);
+    comp.x += (e0 + e1) + (t0 + t1);
+    comp.y += (e2 + e3) + (t2 + t3);
+}
+
 // multiply by e^{S i pi/2}: S=+1 -> i, S=-1 -> -i
 template <int S, typename T>
 __device__ __forceinline__ C2<T> mul_i(C2<T> a) {

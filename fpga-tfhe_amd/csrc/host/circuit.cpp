@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "../hip/engine.h"
@@ -232,9 +233,10 @@ int bind(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec, const int32_t* d_in, 
     return TFHE_OK;
 }
 
-// FP64 launch form per (N, jobs) chosen by timing, shared by every circuit of the process
-std::map<std::pair<int, int64_t>, int>& tuned_forms() {
-    static std::map<std::pair<int, int64_t>, int> m;
+// launch form per (N, precision, bootstraps in the level) chosen by timing, shared by every
+// circuit of the process
+std::map<std::tuple<int, int, int64_t>, int>& tuned_forms() {
+    static std::map<std::tuple<int, int, int64_t>, int> m;
     return m;
 }
 
@@ -287,40 +289,72 @@ int enqueue_all(tfhe_circuit* c, DevTwiddles* tw) {
     return TFHE_OK;
 }
 
-// Times the FP64 launch forms of each level width not seen before: key-sharing workgroups, one
-// bootstrap per workgroup, and for levels of at most 1024 bootstraps the latency form (one
-// bootstrap per workgroup of six wavefronts).  The levels run on whatever the wire storage holds
-// (only the timing is used); the real evaluation follows.
+// Launch forms worth timing for a level of `jobs` bootstraps (BootLaunch::form):
+//   N = 1024, FP64:  1 key-sharing workgroups, 2 one bootstrap per workgroup;
+//   N = 1024, split: 2 one bootstrap per workgroup (its only batched form);
+//   N = 2048:        2, 4, 5 the two-wavefront kernel with 1, 2, 4 bootstraps per workgroup;
+//   and, for levels of at most kLatencyMaxJobs bootstraps, 3 the latency form (one bootstrap per
+//   workgroup of several wavefronts).
+constexpr int64_t kLatencyMaxJobs = 1024;
+constexpr int kTuneSamples = 3;
+
+std::vector<int> tune_candidates(int N, int32_t prec, int64_t jobs) {
+    std::vector<int> c;
+    if (N == 1024) c = prec == TFHE_PREC_FP64 ? std::vector<int>{1, 2} : std::vector<int>{2};
+    else c = {2, 4, 5};
+    if (jobs <= kLatencyMaxJobs) c.push_back(3);
+    return c;
+}
+
+// Times the candidate launch forms of each level width not seen before: per form one untimed
+// launch (warm-up), then kTuneSamples launches each bracketed by events; the form with the lowest
+// median is kept.  The levels run on whatever the wire storage holds (only the timing is used);
+// the real evaluation follows.
 int autotune(tfhe_circuit* c, DevTwiddles* tw) {
     const DevKeyView kv = key_view(c->key);
     hipStream_t st = lib_stream();
     for (LevelPlan& L : c->levels) {
         L.form = 0;
-        if (c->prec != TFHE_PREC_FP64 || kv.p.N != 1024) continue;
         const int64_t jobs = (int64_t)(2 * L.mux.size() + L.other.size()) * c->batch;
         if (!jobs) continue;
-        auto key = std::make_pair(kv.p.N, jobs);
+        const std::vector<int> cand = tune_candidates(kv.p.N, c->prec, jobs);
+        if (cand.size() < 2) {
+            L.form = cand.empty() ? 0 : cand[0];
+            continue;
+        }
+        auto key = std::make_tuple(kv.p.N, (int)c->prec, jobs);
         auto it = tuned_forms().find(key);
         if (it == tuned_forms().end()) {
+            hipEvent_t ev[kTuneSamples + 1];
+            for (hipEvent_t& e : ev)
+                if (hipEventCreate(&e) != hipSuccess) return lib_hip_fail("event", hipErrorUnknown);
             float best = 0.f;
-            int best_form = 1;
-            for (int form : {1, 2, 3}) {
-                if (form == 3 && jobs > 1024) continue;
-                hipEvent_t e0, e1;
-                if (hipEventCreate(&e0) != hipSuccess) return lib_hip_fail("event", hipErrorUnknown);
-                if (hipEventCreate(&e1) != hipSuccess) return lib_hip_fail("event", hipErrorUnknown);
-                (void)hipEventRecord(e0, st);
-                int rc = enqueue_level_boot(c, L, tw, form);
-                (void)hipEventRecord(e1, st);
-                hipError_t e = hipEventSynchronize(e1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                if (rc) return rc;
-                if (e != hipSuccess) return lib_hip_fail("autotune sync", e);
-                if (form == 1 || ms < best) { best = ms; best_form = form; }
+            int best_form = cand[0];
+            int rc = TFHE_OK;
+            for (int form : cand) {
+                if ((rc = enqueue_level_boot(c, L, tw, form))) break;   // warm-up
+                for (int r = 0; r < kTuneSamples && !rc; ++r) {
+                    (void)hipEventRecord(ev[r], st);
+                    rc = enqueue_level_boot(c, L, tw, form);
+                }
+                (void)hipEventRecord(ev[kTuneSamples], st);
+                if (rc) break;
+                hipError_t e = hipEventSynchronize(ev[kTuneSamples]);
+                if (e != hipSuccess) {
+                    rc = lib_hip_fail("autotune sync", e);
+                    break;
+                }
+                float ms[kTuneSamples];
+                for (int r = 0; r < kTuneSamples; ++r) (void)hipEventElapsedTime(&ms[r], ev[r], ev[r + 1]);
+                std::sort(ms, ms + kTuneSamples);
+                const float med = ms[kTuneSamples / 2];
+                if (form == cand[0] || med < best) {
+                    best = med;
+                    best_form = form;
+                }
             }
+            for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
+            if (rc) return rc;
             it = tuned_forms().emplace(key, best_form).first;
         }
         L.form = it->second;
@@ -409,6 +443,16 @@ extern "C" int tfhe_circuit_level_of(tfhe_circuit* c, int32_t wire, int32_t* lev
     if (wire < 0 || wire >= (int32_t)c->nodes.size()) return lib_arg_fail("wire out of range");
     plan(c);
     *level = c->level[wire];
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_circuit_level_plan(tfhe_circuit* c, int32_t level, int32_t* bootstraps, int32_t* form) {
+    if (!c) return lib_arg_fail("null circuit");
+    plan(c);
+    if (level < 1 || level > (int32_t)c->levels.size()) return lib_arg_fail("level out of range");
+    const LevelPlan& L = c->levels[level - 1];
+    if (bootstraps) *bootstraps = (int32_t)(2 * L.mux.size() + L.other.size());
+    if (form) *form = L.form;
     return TFHE_OK;
 }
 

@@ -446,6 +446,20 @@ extern "C" int tfhe_gpu_set_fp64_shared_rows(int32_t on) {
     return TFHE_OK;
 }
 
+extern "C" int tfhe_gpu_set_split_mac(int32_t mode) {
+    if (mode != 0 && mode != 1) return arg_fail("split MAC form must be 0 (plain) or 1 (compensated)");
+    std::lock_guard<std::recursive_mutex> g(ctx().mu);
+    set_split_mac(mode);
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_gpu_set_launch_form(int32_t form) {
+    if (form < 0 || form > 5) return arg_fail("launch form must be in [0, 5]");
+    std::lock_guard<std::recursive_mutex> g(ctx().mu);
+    set_forced_form(form);
+    return TFHE_OK;
+}
+
 extern "C" int tfhe_gpu_set_default_precision(int32_t prec) {
     if (prec != TFHE_PREC_FP64 && prec != TFHE_PREC_SPLIT32) return arg_fail("unknown precision");
     ctx().default_prec = prec;

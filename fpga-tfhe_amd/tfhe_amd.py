@@ -93,6 +93,9 @@ def lib():
         "tfhe_gpu_device_info": (ctypes.c_int, [ctypes.c_char_p, i32, P(i32), P(ctypes.c_int64)]),
         "tfhe_gpu_sync": (ctypes.c_int, []),
         "tfhe_gpu_set_split_bits": (ctypes.c_int, [i32]),
+        "tfhe_gpu_set_split_mac": (ctypes.c_int, [i32]),
+        "tfhe_gpu_set_launch_form": (ctypes.c_int, [i32]),
+        "tfhe_circuit_level_plan": (ctypes.c_int, [_vp, i32, P(i32), P(i32)]),
         "tfhe_gpu_set_default_precision": (ctypes.c_int, [i32]),
         "tfhe_gpu_set_fp64_shared_rows": (ctypes.c_int, [i32]),
         "tfhe_gpu_key_create": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, i32, P(_vp)]),
@@ -236,6 +239,17 @@ def sync():
 
 def set_split_bits(b):
     _ck(lib().tfhe_gpu_set_split_bits(int(b)), "set_split_bits")
+
+
+def set_split_mac(mode):
+    """Split-FP32 accumulation at N=1024: 0 plain FMAs (default), 1 compensated (TwoProduct + TwoSum)."""
+    _ck(lib().tfhe_gpu_set_split_mac(int(mode)), "set_split_mac")
+
+
+def set_launch_form(form):
+    """Bootstrap launch form for launches that name none (0: size rules; 1 FP64 key sharing at
+    N=1024; 2 one bootstrap per workgroup; 3 latency form; 4, 5 N=2048 with 2 / 4 per workgroup)."""
+    _ck(lib().tfhe_gpu_set_launch_form(int(form)), "set_launch_form")
 
 
 def bk_to_lagrange(flat_params, bk):
@@ -493,6 +507,15 @@ class Circuit:
         _ck(lib().tfhe_circuit_info(self.ptr, ctypes.byref(lv), ctypes.byref(mx), ctypes.byref(nb),
                                     ctypes.byref(no)), "circuit_info")
         return {"levels": lv.value, "max_level_bootstraps": mx.value, "bootstraps": nb.value, "outputs": no.value}
+
+    def level_plan(self):
+        """[(bootstraps per instance, launch form of the last run)] for levels 1..n_levels."""
+        out = []
+        for lv in range(1, self.info()["levels"] + 1):
+            nb, form = ctypes.c_int32(), ctypes.c_int32()
+            _ck(lib().tfhe_circuit_level_plan(self.ptr, lv, ctypes.byref(nb), ctypes.byref(form)), "circuit_level_plan")
+            out.append((nb.value, form.value))
+        return out
 
     def level_of(self, wire):
         lv = ctypes.c_int32()

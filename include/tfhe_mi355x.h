@@ -343,6 +343,21 @@ int tfhe_gpu_key_params(const tfhe_gpu_key* key, tfhe_flat_params* out);
 int tfhe_gpu_set_split_bits(int32_t bits);
 /* split point of a device key's SPLIT32 transform (fixed when the key was created) */
 int tfhe_gpu_key_split_bits(const tfhe_gpu_key* key, int32_t* bits);
+/* Accumulation of the split-FP32 pointwise products at N = 1024 (no reference counterpart; the
+ * reference accumulates in FP64, tGswFFTExternMulToTLwe.cpp:172-197): 0 (default) plain FP32 FMAs;
+ * 1 compensated -- each product's rounding error and each addition's are carried in a second FP32
+ * term (TwoProduct by FMA, TwoSum), added back before the inverse transform.  The compensated form
+ * runs one bootstrap per wavefront in every launch size.  Applies to launches issued afterwards. */
+int tfhe_gpu_set_split_mac(int32_t mode);
+/* Launch form of the bootstrap kernels for launches that do not name one (a tuning knob of this
+ * engine, no reference counterpart; 0 restores the size rules):
+ *   1  FP64 at N = 1024: workgroups of 8 bootstraps sharing each key row through LDS;
+ *   2  one bootstrap per workgroup: one wavefront (N = 1024) or a wavefront pair (N = 2048);
+ *   3  latency form: one bootstrap spread over a workgroup of several wavefronts;
+ *   4, 5  N = 2048: 2 or 4 two-wavefront bootstraps per workgroup.
+ * A form that does not apply to a launch's path is treated as 0.  FP64 results are identical in
+ * every form; split-FP32 results differ only in FP32 summation order. */
+int tfhe_gpu_set_launch_form(int32_t form);
 /* precision used by the upstream-API entry points (boots*, tfhe_bootstrap_FFT, ...) */
 int tfhe_gpu_set_default_precision(int32_t prec);
 /* launch form of the FP64 bootstrap (no reference counterpart: a tuning knob of this engine).
@@ -468,12 +483,16 @@ int tfhe_circuit_info(tfhe_circuit* c, int32_t* n_levels, int32_t* max_level_boo
                       int32_t* n_outputs);
 /* bootstrapped level at which a wire is available (0 for inputs and constants) */
 int tfhe_circuit_level_of(tfhe_circuit* c, int32_t wire, int32_t* level);
+/* bootstraps per instance of a bootstrapped level (1 .. n_levels) and the launch form its last
+ * run used (0: the size rules; else as tfhe_gpu_set_launch_form) */
+int tfhe_circuit_level_plan(tfhe_circuit* c, int32_t level, int32_t* bootstraps, int32_t* form);
 /* Evaluates `batch` independent instances.  d_in: device [n_inputs][batch][n+1], d_out: device
  * [n_outputs][batch][n+1].  Asynchronous on the library stream (tfhe_gpu_sync waits).  flags:
  * TFHE_CIRCUIT_GRAPH captures the launch sequence into a HIP graph on the first run of a binding
  * (key, precision, batch, d_in, d_out) and replays it afterwards; TFHE_CIRCUIT_AUTOTUNE times the
- * FP64 launch forms (key-sharing, one bootstrap per workgroup and, up to 1024 bootstraps, the
- * latency form) for every level width not seen before and keeps the fastest.  Runs of one circuit
+ * launch forms of the key's ring size and the precision (tfhe_gpu_set_launch_form lists them; the
+ * latency form only for levels of at most 1024 bootstraps) for every level width not seen before --
+ * one warm-up and three timed launches each -- and keeps the one with the lowest median.  Runs of one circuit
  * must not overlap (its wire storage is shared). */
 int tfhe_circuit_run(tfhe_circuit* c, tfhe_gpu_key* key, int32_t prec, const int32_t* d_in, int32_t* d_out,
                      int32_t batch, int32_t flags);
