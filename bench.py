@@ -49,6 +49,10 @@ def parse():
                     help="MUX and XOR gates each at the N=2048 set (0: skip)")
     ap.add_argument("--split-variants", action="store_true",
                     help="also measure the split-FP32 forms (24/16-bit split x plain/compensated accumulation)")
+    ap.add_argument("--fused-ks-ab", action="store_true",
+                    help="also time the keyswitch fused into the split bootstrap kernel against the separate kernel")
+    ap.add_argument("--form-sweep", action="store_true",
+                    help="also time every bootstrap launch form on the headline and N=2048 workloads")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -227,6 +231,79 @@ def split_variants_report(T, ks, batch, steps, audit_gates, rng):
         for b_ in (d_a, d_b, d_out):
             b_.free()
     return out
+
+
+def form_sweep_report(T, dk, bufs, batch, n2048_gates):
+    """Every launch form that applies, forced in turn: the headline NAND step at N=1024 (FP64: key
+    sharing, one bootstrap per workgroup; split: one per workgroup) and one MUX + XOR step at N=2048
+    (1, 2 or 4 two-wavefront bootstraps per workgroup), median of 3 timed steps after a warm-up."""
+    out = {"n1024": {}, "n2048": {}}
+    d_out, d_a, d_b = bufs
+    try:
+        for name, prec, forms in (("fp64", T.PREC_FP64, (1, 2)), ("split32", T.PREC_SPLIT32, (2,))):
+            for f in forms:
+                T.set_launch_form(f)
+                ts = []
+                for rep in range(4):
+                    T.sync()
+                    t0 = time.perf_counter()
+                    dk.gate_dev("NAND", d_out.ptr, d_a.ptr, d_b.ptr, None, batch, prec=prec)
+                    T.sync()
+                    ts.append((time.perf_counter() - t0) * 1e3)
+                out["n1024"][f"{name} {FORM_NAMES[f]}"] = round(sorted(ts[1:])[1], 2)
+        if n2048_gates > 0:
+            ks2 = T.KeySet(2048, seed=(0,))
+            dk2 = T.DeviceKey.from_keyset(ks2)
+            G = n2048_gates
+            rng = np.random.default_rng(9)
+            bits = rng.integers(0, 2, (3, G)).astype(np.int32)
+            bb = [T.DeviceBuffer.from_array(ks2.encrypt(bits[i])) for i in range(3)]
+            oo = [T.DeviceBuffer(G * (ks2.fp.n + 1) * 4) for _ in range(2)]
+            try:
+                for name, prec in (("fp64", T.PREC_FP64), ("split32", T.PREC_SPLIT32)):
+                    for f in (2, 4, 5):
+                        T.set_launch_form(f)
+                        ts = []
+                        for rep in range(4):
+                            T.sync()
+                            t0 = time.perf_counter()
+                            dk2.gate_dev("MUX", oo[0].ptr, bb[0].ptr, bb[1].ptr, bb[2].ptr, G, prec=prec)
+                            dk2.gate_dev("XOR", oo[1].ptr, bb[0].ptr, bb[1].ptr, None, G, prec=prec)
+                            T.sync()
+                            ts.append((time.perf_counter() - t0) * 1e3)
+                        ok = bool(np.array_equal(T.decrypt_dev(oo[0], ks2.lwe_key, G), np.where(bits[0] == 1, bits[1], bits[2])))
+                        out["n2048"][f"{name} {FORM_NAMES[f]}"] = {"ms_per_step": round(sorted(ts[1:])[1], 2), "correct": ok}
+            finally:
+                for b_ in bb + oo:
+                    b_.free()
+                dk2.free()
+                ks2.free()
+    finally:
+        T.set_launch_form(0)
+    log(f"[bench] launch-form sweep: {out}")
+    return out
+
+
+def fused_ks_report(T, dk, bufs, batch, steps, want, ks):
+    """Same-box A/B of the headline NAND step on the split path: bootstrap kernel + separate
+    keyswitch kernel against the keyswitch fused into the bootstrap kernel, alternating A, B, A, B;
+    the two outputs must be bit-identical."""
+    d_out, d_a, d_b = bufs
+    nn = ks.fp.n
+    res = {"separate": [], "fused": []}
+    outs = {}
+    try:
+        for rep in range(2):
+            for name, on in (("separate", False), ("fused", True)):
+                T.set_fused_keyswitch(on)
+                r = time_path(T, dk, T.PREC_SPLIT32, bufs, batch, steps, 1)
+                res[name].append(round(r["ms_per_step"], 3))
+                outs[name] = d_out.to_array((batch, nn + 1))
+    finally:
+        T.set_fused_keyswitch(False)
+    return {"ms_per_step": res, "bit_identical": bool(np.array_equal(outs["separate"], outs["fused"])),
+            "fused_correct": bool(np.array_equal(ks.decrypt(outs["fused"]), want)),
+            "workload": f"{batch} NAND, split-FP32, N=1024"}
 
 
 def adders_report(T, dk, ks, count, reps=2):
@@ -492,6 +569,11 @@ def main():
     prec_rep = precision_report(T, dk, np.random.default_rng(3)) if len(precs) == 2 else None
     log(f"[bench] split32 external-product error {prec_rep}")
 
+    fused = (fused_ks_report(T, dk, (d_out, d_a, d_b), B, max(2, args.steps // 2), want, ks)
+             if args.fused_ks_ab else None)
+    if fused:
+        log(f"[bench] fused keyswitch A/B: {fused}")
+    sweep = form_sweep_report(T, dk, (d_out, d_a, d_b), B, args.n2048_gates) if args.form_sweep else None
     variants = (split_variants_report(T, ks, B, max(2, args.steps // 2), args.audit_gates, np.random.default_rng(4))
                 if args.split_variants else None)
     adders = adders_report(T, dk, ks, args.adders) if args.adders > 0 and args.prec == "both" else None
@@ -554,6 +636,8 @@ def main():
                      "key_stream": key_stream},
         "cpu_baseline": cpu,
         "split32_variants": variants,
+        "fused_keyswitch_ab": fused,
+        "launch_form_sweep": sweep,
         "circuits_32bit_adders": adders,
         "n2048": n2048,
         "device": dev,

@@ -80,6 +80,10 @@ struct BootLaunch {
     double* raw_out = nullptr;
     double* raw_hi = nullptr;   // mode 2, split FP32: the hi product before rounding (units of 2^split)
     int32_t* u_out = nullptr;
+    // fused keyswitch (split FP32, N = 1024, mode 0, one segment): job g writes the keyswitched
+    // sample KS((0, ks_add_b) + extract(acc_g)) to ks_out[g] instead of the extracted sample
+    int32_t* ks_out = nullptr;
+    int32_t ks_add_b = 0;
 };
 
 struct DevKeyView {
@@ -110,6 +114,9 @@ void set_boot64_shared(int mode);
 void set_split_mac(int mode);
 // launch form for launches that name none (0 restores the size rules)
 void set_forced_form(int form);
+// 1: split-FP32 gate batches at N = 1024 (not MUX) keyswitch inside the bootstrap kernel
+void set_fused_keyswitch(int on);
+int fused_keyswitch();
 hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& L,
                             hipStream_t st);
 // KS((0, add_b) + u1 + u2) for each of count samples; u2 may be null.  Output g goes to out[g], or,

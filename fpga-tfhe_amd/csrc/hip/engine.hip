@@ -30,6 +30,7 @@
 namespace tfhe {
 
 constexpr int kMaxLweN = 1024;   // largest LWE dimension the LDS prologue holds
+constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2) in [0,4)^2 without (0, 0)
 
 // Resident wavefronts per SIMD the bootstrap kernels are compiled for (register budget 512/W).
 #ifndef TFHE_BOOT64_WAVES
@@ -51,6 +52,7 @@ struct BootKArgs {
     const void* t2w_rw32;   // and FP32
     int n;
     int split_bits;
+    const int32_t* ksk;     // digit-pair keyswitch key (fused keyswitch form)
 };
 
 template <int N>
@@ -660,9 +662,52 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
     wave_sync();
 }
 
+// Fused keyswitch (KSF): the wavefront keyswitches its own extracted sample straight from the
+// accumulator in LDS instead of writing the sample out for k_keyswitch.  Extraction: u_0 = a_0,
+// u_j = -a_{N-j}, u_N = b_0; keyswitch as k_keyswitch (digit pairs, lwe-keyswitch-functions.cpp:
+// 114-129, 228-241), bit-identical.  Lane L owns output coefficients CPL*L .. CPL*L+CPL-1 of the
+// NP-padded rows, read with 8-byte loads; the digit pair is uniform, and a zero pair reads row 0
+// under a zero mask, so the loop has no branches and its loads can be issued ahead.
+template <int TKS, int KN, int NP>
+__device__ __forceinline__ void keyswitch_from_acc(const int32_t* s_acc, const int32_t* __restrict__ ksk,
+                                                   int32_t* __restrict__ out, int n, int lane, int32_t add_b) {
+    constexpr int TP = TKS / 2, RPI = TP * KS_PAIR_ROWS, CPL = NP / 64;
+    static_assert(NP % 128 == 0 || CPL % 2 == 0, "8-byte row reads");
+    const uint32_t prec_off = 1u << (32 - (1 + 2 * TKS));
+    uint32_t acc[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) acc[t] = 0;
+    const int32_t* kl = ksk + (size_t)CPL * lane;
+#pragma unroll 2
+    for (int i = 0; i < KN; ++i) {
+        const uint32_t ui = (i == 0) ? (uint32_t)s_acc[0] : 0u - (uint32_t)s_acc[KN - i];
+        const uint32_t abar = (uint32_t)__builtin_amdgcn_readfirstlane((int)(ui + prec_off));
+#pragma unroll
+        for (int jp = 0; jp < TP; ++jp) {
+            const uint32_t d = (abar >> (28 - 4 * jp)) & 15u;
+            const uint32_t m = d ? 0xFFFFFFFFu : 0u;
+            const int row = d ? jp * KS_PAIR_ROWS + (int)d - 1 : 0;
+            const int2* kr = (const int2*)(kl + ((size_t)i * RPI + row) * NP);
+#pragma unroll
+            for (int t = 0; t < CPL / 2; ++t) {
+                const int2 v = kr[t];
+                acc[2 * t] -= (uint32_t)v.x & m;
+                acc[2 * t + 1] -= (uint32_t)v.y & m;
+            }
+        }
+    }
+    const uint32_t bterm = (uint32_t)s_acc[KN] + (uint32_t)add_b;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int c = CPL * lane + t;
+        if (c <= n) out[c] = (int32_t)(acc[t] + (c == n ? bterm : 0u));
+    }
+}
+
 // MAC = 1 (compensated accumulation) doubles the accumulator registers and is compiled for one
-// wavefront per SIMD.
-template <int N, int L, int BG, int MAC = 0>
+// wavefront per SIMD.  KSF: fused keyswitch (mode 0, one segment, N = 1024 set), output to
+// L.ks_out[g] instead of an extracted sample at L.u_out[g].
+template <int N, int L, int BG, int MAC = 0, bool KSF = false>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1,
                                    N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1))) k_boot32(BootKArgs A) {
@@ -701,7 +746,11 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1
                                         nullptr, nullptr);
         }
     }
-    boot_epilogue<N>(A, g, lane, s_acc);
+    if constexpr (KSF) {
+        keyswitch_from_acc<8, N, 640>(s_acc, A.ksk, A.L.ks_out + (size_t)g * (A.n + 1), A.n, lane, A.L.ks_add_b);
+    } else {
+        boot_epilogue<N>(A, g, lane, s_acc);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -816,7 +865,6 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
 // ---------------------------------------------------------------------------------------------
 constexpr int KS_GATES = 256;
 constexpr int KS_IBLK = 16;
-constexpr int KS_PAIR_ROWS = 15;   // (d1, d2) in [0,4)^2 without (0, 0)
 
 template <int TKS, int KN, int NP, int C>
 __global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
@@ -2165,6 +2213,11 @@ void set_split_mac(int mode) { g_split_mac = mode; }
 static int g_forced_form = 0;
 void set_forced_form(int form) { g_forced_form = form; }
 
+// keyswitch fused into the split-FP32 bootstrap kernel (an evaluated alternative; off by default)
+static int g_fused_ks = 0;
+void set_fused_keyswitch(int on) { g_fused_ks = on; }
+int fused_keyswitch() { return g_fused_ks; }
+
 static int device_cus() {
     static int cus = 0;
     if (!cus) {
@@ -2190,7 +2243,7 @@ static int boot64_single_round() { return 4 * device_cus(); }
 template <int N, int L, int BG>
 static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Prec prec, const BootLaunch& Lc,
                                 hipStream_t st) {
-    BootKArgs A;
+    BootKArgs A{};
     A.L = Lc;
     A.n = key.p.n;
     A.split_bits = key.split_bits;
@@ -2229,6 +2282,14 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
                 hipLaunchKernelGGL((k_boot32<N, L, BG, 1>), dim3(Lc.count), dim3(64), 0, st, A);
                 return hipGetLastError();
             }
+            if (Lc.ks_out) {
+                // fused keyswitch: mode 0, one gate segment, the (t, NP) = (8, 640) keyswitch key
+                if (Lc.mode != 0 || Lc.nseg != 1 || key.ksk_np != 640 || key.p.ks_t != 8 || !key.ksk)
+                    return hipErrorInvalidValue;
+                A.ksk = key.ksk;
+                hipLaunchKernelGGL((k_boot32<N, L, BG, 0, true>), dim3(Lc.count), dim3(64), 0, st, A);
+                return hipGetLastError();
+            }
             // forms: 3 the latency form, 2 one bootstrap per wavefront, 0 (and the forms of other
             // paths) by launch size
             int form = Lc.form ? Lc.form : g_forced_form;
@@ -2258,12 +2319,15 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
         A.t2w_rw32 = tw.rw32;
         // launches of at most one bootstrap per CU take the latency form (FP64: unless a launch form
         // other than it is forced, as at N = 1024).  Forms: 3 latency; 2, 4, 5 the batched kernel
-        // with 1, 2, 4 bootstraps per workgroup; 0 by launch size (FP64 4, split 2 per workgroup).
+        // with 1, 2, 4 bootstraps per workgroup; 0 by launch size (2 per workgroup).  Two per
+        // workgroup, two workgroups per CU, beat one (the rows' reads no longer meet in L1) and four
+        // (a longer barrier lockstep): per step of 8192 MUX + 8192 XOR on one box, FP64 652 / 482 /
+        // 502 ms and split 331 / 306 / 328 ms for 1 / 2 / 4.
         const int form = Lc.form ? Lc.form : g_forced_form;
         const bool lat = Lc.mode != 2 && A.n <= kMaxLweN &&
                          (form == 3 || (form == 0 && (prec == PREC_SPLIT32 || g_boot64_shared == 2) &&
                                         Lc.count <= boot64_latency_max()));
-        const int gb = form == 2 ? 1 : form == 4 ? 2 : form == 5 ? 4 : (prec == PREC_FP64 ? 4 : 2);
+        const int gb = form == 2 ? 1 : form == 5 ? 4 : 2;
         if (prec == PREC_FP64) {
             A.bk = key.bk64;
             A.t1 = tw.t1w64;
@@ -2272,8 +2336,6 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
                 hipLaunchKernelGGL((k_boot2wlat<3, 8, 0>), dim3(Lc.count), dim3(64 * 6), 0, st, A);
                 return hipGetLastError();
             }
-            // default 4 bootstraps (8 wavefronts): one workgroup per CU at two wavefronts per SIMD
-            // (two bootstraps per workgroup measured 1% slower here)
             if (gb == 1) hipLaunchKernelGGL((k_boot2w<3, 8, 0, 1>), dim3(Lc.count), dim3(128), 0, st, A);
             else if (gb == 2) hipLaunchKernelGGL((k_boot2w<3, 8, 0, 2>), dim3((Lc.count + 1) / 2), dim3(256), 0, st, A);
             else hipLaunchKernelGGL((k_boot2w<3, 8, 0, 4>), dim3((Lc.count + 3) / 4), dim3(512), 0, st, A);
@@ -2286,10 +2348,6 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
                 hipLaunchKernelGGL((k_boot2wlat<3, 8, 1>), dim3(Lc.count), dim3(64 * 8), 0, st, A);
                 return hipGetLastError();
             }
-            // default two bootstraps (4 wavefronts) per workgroup, two workgroups per CU at two
-            // wavefronts per SIMD: fewer wavefronts per barrier than with four per workgroup (3.8%
-            // faster on 4096 MUX + 4096 XOR), while each row's reads still meet in L1 (one per
-            // workgroup: 14% slower)
             if (gb == 1) hipLaunchKernelGGL((k_boot2w<3, 8, 1, 1>), dim3(Lc.count), dim3(128), 0, st, A);
             else if (gb == 4) hipLaunchKernelGGL((k_boot2w<3, 8, 1, 4>), dim3((Lc.count + 3) / 4), dim3(512), 0, st, A);
             else hipLaunchKernelGGL((k_boot2w<3, 8, 1, 2>), dim3((Lc.count + 1) / 2), dim3(256), 0, st, A);

@@ -266,6 +266,19 @@ int gate_batch_dev_impl(tfhe_gpu_key* key, int32_t prec, int32_t gate, int32_t* 
     L.count = nboot;
     L.mu = MU;
     L.u_out = u;
+    if (fused_keyswitch() && prec == TFHE_PREC_SPLIT32 && N == 1024 && gate != TFHE_GATE_MUX) {
+        // one launch: the bootstrap kernel keyswitches its own sample into the output
+        L.nseg = 1;
+        L.seg_start[1] = count;
+        L.in_a[0] = a;
+        L.in_b[0] = b;
+        gate_coeffs(gate, &L.cst[0], &L.alpha[0], &L.beta[0]);
+        L.ks_out = out;
+        std::unique_ptr<ProfScope> ps(prof ? new ProfScope(cx, "bootstrap_split32_fused_keyswitch") : nullptr);
+        hipError_t e = launch_bootstrap(key->view(), *tw, (Prec)prec, L, cx.stream);
+        if (e != hipSuccess) return hip_fail("fused bootstrap+keyswitch launch", e);
+        return TFHE_OK;
+    }
     if (gate != TFHE_GATE_MUX) {
         L.nseg = 1;
         L.seg_start[1] = count;
@@ -450,6 +463,13 @@ extern "C" int tfhe_gpu_set_split_mac(int32_t mode) {
     if (mode != 0 && mode != 1) return arg_fail("split MAC form must be 0 (plain) or 1 (compensated)");
     std::lock_guard<std::recursive_mutex> g(ctx().mu);
     set_split_mac(mode);
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_gpu_set_fused_keyswitch(int32_t on) {
+    if (on != 0 && on != 1) return arg_fail("fused keyswitch flag must be 0 or 1");
+    std::lock_guard<std::recursive_mutex> g(ctx().mu);
+    set_fused_keyswitch(on);
     return TFHE_OK;
 }
 

@@ -95,6 +95,7 @@ def lib():
         "tfhe_gpu_set_split_bits": (ctypes.c_int, [i32]),
         "tfhe_gpu_set_split_mac": (ctypes.c_int, [i32]),
         "tfhe_gpu_set_launch_form": (ctypes.c_int, [i32]),
+        "tfhe_gpu_set_fused_keyswitch": (ctypes.c_int, [i32]),
         "tfhe_circuit_level_plan": (ctypes.c_int, [_vp, i32, P(i32), P(i32)]),
         "tfhe_gpu_set_default_precision": (ctypes.c_int, [i32]),
         "tfhe_gpu_set_fp64_shared_rows": (ctypes.c_int, [i32]),
@@ -250,6 +251,12 @@ def set_launch_form(form):
     """Bootstrap launch form for launches that name none (0: size rules; 1 FP64 key sharing at
     N=1024; 2 one bootstrap per workgroup; 3 latency form; 4, 5 N=2048 with 2 / 4 per workgroup)."""
     _ck(lib().tfhe_gpu_set_launch_form(int(form)), "set_launch_form")
+
+
+def set_fused_keyswitch(on):
+    """Split-FP32 gate batches at N=1024 (not MUX): keyswitch inside the bootstrap kernel (True) or in
+    the separate keyswitch kernel (False, the default)."""
+    _ck(lib().tfhe_gpu_set_fused_keyswitch(1 if on else 0), "set_fused_keyswitch")
 
 
 def bk_to_lagrange(flat_params, bk):

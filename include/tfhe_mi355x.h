@@ -358,6 +358,11 @@ int tfhe_gpu_set_split_mac(int32_t mode);
  * A form that does not apply to a launch's path is treated as 0.  FP64 results are identical in
  * every form; split-FP32 results differ only in FP32 summation order. */
 int tfhe_gpu_set_launch_form(int32_t form);
+/* 1: split-FP32 gate batches at N = 1024 (all gates but MUX) keyswitch inside the bootstrap kernel,
+ * from the accumulator in LDS, instead of in a separate keyswitch launch (an evaluated alternative
+ * to the separate kernel, no reference counterpart; outputs are bit-identical to the separate
+ * kernel's for the same accumulators).  0 (default): separate keyswitch kernel. */
+int tfhe_gpu_set_fused_keyswitch(int32_t on);
 /* precision used by the upstream-API entry points (boots*, tfhe_bootstrap_FFT, ...) */
 int tfhe_gpu_set_default_precision(int32_t prec);
 /* launch form of the FP64 bootstrap (no reference counterpart: a tuning knob of this engine).
