@@ -53,6 +53,8 @@ def parse():
                     help="also time the keyswitch fused into the split bootstrap kernel against the separate kernel")
     ap.add_argument("--form-sweep", action="store_true",
                     help="also time every bootstrap launch form on the headline and N=2048 workloads")
+    ap.add_argument("--launch-form", type=int, default=0,
+                    help="force a bootstrap launch form for every launch (0: the size rules; see tfhe_gpu_set_launch_form)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -507,6 +509,8 @@ def main():
         sys.exit(2)
     import tfhe_amd as T
     T.init(0)
+    if args.launch_form:
+        T.set_launch_form(args.launch_form)   # before the keys are made: form 6 needs its key layout
     dev = T.device_info()
     log(f"[bench] device {dev}")
     t = time.perf_counter()

@@ -29,6 +29,8 @@ struct DevTwiddles {
     void* rw64 = nullptr;        // [2][8] C2<double>: w^{64 a (1 + 4v)}
     void* rw32 = nullptr;        // [2][8] C2<float>
     void* rws64 = nullptr;       // [2][8] C2<double>: w^{64 a (1 + 4v)} / 1024 (split path, own untwist)
+    // N = 1024, 4 bootstraps per wavefront (form 6): pass-2 twiddles [16][2][16] C2<float>
+    void* q4tw = nullptr;
 };
 
 // One gate segment of a table-driven bootstrap launch (mode 3): jobs [s*seg_jobs, (s+1)*seg_jobs)
@@ -59,7 +61,9 @@ struct LinDesc {
 //   2 = one bootstrap per workgroup: k_boot64 / k_boot32 (N=1024), k_boot2w with one two-wavefront
 //       bootstrap per workgroup (N=2048);
 //   3 = latency form, one bootstrap per workgroup of several wavefronts (modes 0, 1, 3);
-//   4, 5 = N=2048: k_boot2w with 2 or 4 bootstraps per workgroup.
+//   4, 5 = N=2048: k_boot2w with 2 or 4 bootstraps per workgroup;
+//   6 = split FP32 at N=1024: k_boot32q, 4 bootstraps per wavefront with the products on the
+//       matrix cores (modes 0, 1, 3; needs the key's bk32q).
 // A form that does not apply to a launch's path (1 on the split path or at N=2048, 4 and 5 at
 // N=1024) is treated as 0.
 struct BootLaunch {
@@ -90,6 +94,7 @@ struct DevKeyView {
     DevParams p;
     const void* bk64 = nullptr;      // FP64 FFT-domain key, lane order
     const void* bk32 = nullptr;      // split FP32 key {hi.re, hi.im, lo.re, lo.im}
+    const void* bk32q = nullptr;     // the split key arranged for form 6 (N = 1024; may be null)
     int split_bits = 24;
     const int32_t* ksk = nullptr;    // [kN][t/2][15][NP] digit-pair rows
     int ksk_np = 0;                  // padded output length (multiple of 64)
@@ -114,6 +119,7 @@ void set_boot64_shared(int mode);
 void set_split_mac(int mode);
 // launch form for launches that name none (0 restores the size rules)
 void set_forced_form(int form);
+int forced_form();
 // 1: split-FP32 gate batches at N = 1024 (not MUX) keyswitch inside the bootstrap kernel
 void set_fused_keyswitch(int on);
 int fused_keyswitch();
@@ -130,6 +136,11 @@ hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32
 // KSK [kN][t][base][n+1] (device) -> [kN][t/2][15][NP]: row 4 d1 + d2 - 1 of pair g holds
 // ks[2g][d1] + ks[2g+1][d2] (base 4 only)
 size_t ksk_device_bytes(const DevParams& p, int np);
+// split key bk32 -> the layout of launch form 6 (N = 1024, l = 3): n * 6 * 32 * 64 float4
+hipError_t launch_keyprep_q4(const DevParams& p, const void* bk32, void* bk32q, hipStream_t st);
+// transform test of form 6's 4-per-wavefront layout (N = 1024), semantics of launch_fft_test
+hipError_t launch_fft_test_q4(const DevTwiddles& tw, int inverse, double* out, const void* in, int count,
+                              hipStream_t st);
 hipError_t launch_ksk_relayout(const DevParams& p, const int32_t* d_ksk_in, int32_t* d_ksk_out, int np,
                                hipStream_t st);
 hipError_t launch_lwe_linear(const LinDesc* d_descs, int ndesc, int count, int n, hipStream_t st);

@@ -1358,6 +1358,19 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
             }
         }
     }
+    // 4-bootstraps-per-wavefront form (N = 1024): pass-2 twiddles T[i][l + 16 s] =
+    // e^{i pi i (4 (l + 16 s) + 1) / N}, table [l < 16][s < 2][i < 16]
+    std::vector<float> q4(2 * 16 * 2 * 16);
+    if (N == 1024) {
+        for (int l = 0; l < 16; ++l)
+            for (int s = 0; s < 2; ++s)
+                for (int i = 0; i < 16; ++i) {
+                    double re, im;
+                    cis_pi_over_N((long long)i * (4 * (l + 16 * s) + 1), N, &re, &im);
+                    const size_t o = 2 * ((size_t)(l * 2 + s) * 16 + i);
+                    q4[o] = (float)re; q4[o + 1] = (float)im;
+                }
+    }
     hipError_t e;
     auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
         hipError_t r = hipMalloc(dst, bytes);
@@ -1373,6 +1386,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     if ((e = up(&tw.twist32, tws.data(), tws.size() * 4)) != hipSuccess) return e;
     if ((e = up(&tw.untw64, untw.data(), untw.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
+    if (N == 1024 && (e = up(&tw.q4tw, q4.data(), q4.size() * 4)) != hipSuccess) return e;
     if (N == 2048) {
         if ((e = up(&tw.t1w64, t1w.data(), t1w.size() * 8)) != hipSuccess) return e;
         if ((e = up(&tw.t1w32, t1w32.data(), t1w32.size() * 4)) != hipSuccess) return e;
@@ -1385,7 +1399,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
 
 void free_twiddles(DevTwiddles& tw) {
     void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.t2dit32, tw.twist32, tw.untw64, tw.wa64,
-                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64};
+                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64, tw.q4tw};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -2177,6 +2191,448 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
     }
 }
 
+// =============================================================================================
+// Split-FP32 bootstrap with the pointwise products on the matrix cores (launch form 6, N = 1024).
+//
+// The per-frequency products of the external product, out[o][f] += dec[f] * K[o][f], use the same
+// key value for every bootstrap, so they form a contraction (and can run on the matrix cores) only
+// across bootstraps.  A wavefront here runs 4 bootstraps: lane 4 l + b works on bootstrap b
+// (l < 16), and each bootstrap's 512-point transform is spread over its 16 lanes, 32 points each:
+//   forward  lane l, register r holds z_j, j = 16 r + l, taken in as (d_j + i d_{j+512}) w^{16 r}
+//            (w = e^{i pi / 1024}); pass 1: a 32-point DFT over r gives Y_l[k1]; pass-2 twiddle
+//            T[l][k1] = e^{i pi l (4 k1 + 1) / 1024}, which also carries the lane part w^l of the
+//            twist; one LDS exchange gives lane l' the values k1 = l' + 16 s of every l; pass 2: a
+//            16-point DFT over l for each s.  Register 16 s + k2 of lane l' then holds F_f,
+//            f = l' + 16 s + 32 k2, in the transform convention of the other kernels.
+//   products for every register, v_mfma_f32_4x4x1_16b_f32 with block l' (16 frequencies), rows =
+//            the four components (hi.re, hi.im, lo.re, lo.im) of one output polynomial's key value
+//            and columns = the four bootstraps: D[c][4 l' + b] += A[4 l' + c] * B[4 l' + b], with B
+//            the real or imaginary part of the digit spectrum (the register itself) and A the key
+//            arranged for it: K's components for the real part, (-K.im, K.re) per half for the
+//            imaginary part.  The sums land in the transform's own layout, lane 4 l' + b.
+//   inverse  the passes in reverse with conjugate twiddles; the uniform untwist conj(w^{16 r}) /
+//            512 and the rounding run in FP64, as in the other split kernels.
+// Key (k_keyprep_q4): per LWE index, row and register one float4 per lane, {A_re(K_0), A_im(K_0),
+// A_re(K_1), A_im(K_1)} (component lane & 3 of each), 1 KB per register, 192 KB per row set; one
+// load serves four bootstraps.
+// LDS: the 4 accumulators (32 KB) and a 4.25 KB exchange buffer: four wavefronts per CU, one per
+// SIMD.  The modulus-switched rotations are recomputed from the inputs at each step.
+// =============================================================================================
+// e^{i pi r / 64}, r < 32: the uniform part w^{16 r} of the forward twist (w = e^{i pi / 1024})
+constexpr float kQ4TwRe[32] = {
+    1.000000000e+00f, 9.987954562e-01f, 9.951847267e-01f, 9.891765100e-01f, 9.807852804e-01f, 9.700312532e-01f,
+    9.569403357e-01f, 9.415440652e-01f, 9.238795325e-01f, 9.039892931e-01f, 8.819212643e-01f, 8.577286100e-01f,
+    8.314696123e-01f, 8.032075315e-01f, 7.730104534e-01f, 7.409511254e-01f, 7.071067812e-01f, 6.715589548e-01f,
+    6.343932842e-01f, 5.956993045e-01f, 5.555702330e-01f, 5.141027442e-01f, 4.713967368e-01f, 4.275550934e-01f,
+    3.826834324e-01f, 3.368898534e-01f, 2.902846773e-01f, 2.429801799e-01f, 1.950903220e-01f, 1.467304745e-01f,
+    9.801714033e-02f, 4.906767433e-02f
+};
+constexpr float kQ4TwIm[32] = {
+    0.000000000e+00f, 4.906767433e-02f, 9.801714033e-02f, 1.467304745e-01f, 1.950903220e-01f, 2.429801799e-01f,
+    2.902846773e-01f, 3.368898534e-01f, 3.826834324e-01f, 4.275550934e-01f, 4.713967368e-01f, 5.141027442e-01f,
+    5.555702330e-01f, 5.956993045e-01f, 6.343932842e-01f, 6.715589548e-01f, 7.071067812e-01f, 7.409511254e-01f,
+    7.730104534e-01f, 8.032075315e-01f, 8.314696123e-01f, 8.577286100e-01f, 8.819212643e-01f, 9.039892931e-01f,
+    9.238795325e-01f, 9.415440652e-01f, 9.569403357e-01f, 9.700312532e-01f, 9.807852804e-01f, 9.891765100e-01f,
+    9.951847267e-01f, 9.987954562e-01f
+};
+// the same in FP64 with the inverse transform's 1/512 folded in (the untwist multiplies by the conjugate)
+constexpr double kQ4UntwRe[32] = {
+    1.95312500000000000e-03, 1.95077237540072735e-03, 1.94372016928163463e-03, 1.93198537102496292e-03,
+    1.91559625078755943e-03, 1.89459229139559370e-03, 1.86902409322697036e-03, 1.83895325231058751e-03,
+    1.80445221193610691e-03, 1.76560408813172527e-03, 1.72250246943038096e-03, 1.67525119140678148e-03,
+    1.62396408652840866e-03, 1.56876470992313465e-03, 1.50978604172409569e-03, 1.44717016670890450e-03,
+    1.38106793200497573e-03, 1.31163858368558268e-03, 1.23904938313212009e-03, 1.16347520408678412e-03,
+    1.08509811136641072e-03, 1.00410692225238606e-03, 9.20696751613276970e-04, 8.35068541856019914e-04,
+    7.47428578838066088e-04, 6.57987994906679787e-04, 5.66962260262621740e-04, 4.74570663873562465e-04,
+    3.81035785187750647e-04, 2.86582957920628414e-04, 1.91439727206173379e-04, 9.58353014207385273e-05
+};
+constexpr double kQ4UntwIm[32] = {
+    0.00000000000000000e+00, 9.58353014207383104e-05, 1.91439727206173054e-04, 2.86582957920628414e-04,
+    3.81035785187750485e-04, 4.74570663873562248e-04, 5.66962260262621740e-04, 6.57987994906679787e-04,
+    7.47428578838065980e-04, 8.35068541856019697e-04, 9.20696751613276645e-04, 1.00410692225238606e-03,
+    1.08509811136641050e-03, 1.16347520408678390e-03, 1.23904938313212009e-03, 1.31163858368558268e-03,
+    1.38106793200497551e-03, 1.44717016670890450e-03, 1.50978604172409569e-03, 1.56876470992313444e-03,
+    1.62396408652840866e-03, 1.67525119140678148e-03, 1.72250246943038074e-03, 1.76560408813172527e-03,
+    1.80445221193610691e-03, 1.83895325231058751e-03, 1.86902409322697058e-03, 1.89459229139559370e-03,
+    1.91559625078755943e-03, 1.93198537102496292e-03, 1.94372016928163441e-03, 1.95077237540072735e-03
+};
+// W_32^m = e^{2 pi i m / 32}, m < 32
+constexpr float kW32Re[32] = {
+    1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f,
+    3.826834324e-01f, 1.950903220e-01f, 0.000000000e+00f, -1.950903220e-01f, -3.826834324e-01f, -5.555702330e-01f,
+    -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f, -9.807852804e-01f, -1.000000000e+00f,
+    -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f, -7.071067812e-01f, -5.555702330e-01f,
+    -3.826834324e-01f, -1.950903220e-01f, 0.000000000e+00f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f,
+    7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f, 9.807852804e-01f
+};
+constexpr float kW32Im[32] = {
+    0.000000000e+00f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f, 7.071067812e-01f, 8.314696123e-01f,
+    9.238795325e-01f, 9.807852804e-01f, 1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f,
+    7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f, 1.950903220e-01f, 0.000000000e+00f, -1.950903220e-01f,
+    -3.826834324e-01f, -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f, -9.238795325e-01f,
+    -9.807852804e-01f, -1.000000000e+00f, -9.807852804e-01f, -9.238795325e-01f, -8.314696123e-01f,
+    -7.071067812e-01f, -5.555702330e-01f, -3.826834324e-01f, -1.950903220e-01f
+};
+typedef float q4v4f __attribute__((ext_vector_type(4)));
+constexpr int Q4_NB = 4;                       // bootstraps per wavefront
+constexpr uint32_t Q4_ROWSET_B = 6u * 32u * 64u * 16u;   // bytes of one LWE index's row set (l = 3)
+
+template <int S>
+__device__ __forceinline__ void q4_dft4(C2<float>& x0, C2<float>& x1, C2<float>& x2, C2<float>& x3) {
+    const C2<float> a0 = x0 + x2, a1 = x0 - x2, b0 = x1 + x3, b1 = mul_i<S>(x1 - x3);
+    x0 = a0 + b0;
+    x2 = a0 - b0;
+    x1 = a1 + b1;
+    x3 = a1 - b1;
+}
+
+// X_k at x[q4_slot(k)] after q4_dft32
+__host__ __device__ constexpr int q4_slot(int k) { return 8 * (k & 3) + (k >> 2); }
+
+// 32-point DFT X_k = sum_n x_n e^{S 2 pi i n k / 32}, natural order in: n = 8 n1 + n2, k = k1 + 4 k2;
+// 4-point DFTs over n1, twiddles W_32^{S n2 k1} fused into the 8-point DFTs over n2
+template <int S>
+__device__ __forceinline__ void q4_dft32(C2<float>* x) {
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) q4_dft4<S>(x[n2], x[8 + n2], x[16 + n2], x[24 + n2]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        C2<float> w[8];
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) w[n2] = C2<float>{kW32Re[(n2 * k1) & 31], kW32Im[(n2 * k1) & 31]};
+        dft8_tw<S, (S < 0), true>(x + 8 * k1, w);
+    }
+}
+
+// The exchange moves the values of one s-half (k1 = c + 16 s, c < 16) at a time, real parts then
+// imaginary parts: element (c, source lane x = 4 l + b) at 68 c + x.  Writes (one c, every lane)
+// and reads (one source l of c = l') meet 64 distinct banks, and both addresses are a per-lane
+// base plus an immediate offset: 1088 floats (4352 B) of buffer.
+constexpr int Q4_XROW = 68, Q4_XBUF = 16 * Q4_XROW;
+
+// v[q4_slot(k1)] = Y[k1] (k1 < 32) -> v[q4_slot(16 s + l)] = Y_l[l' + 16 s] (this lane is l'): each
+// half s returns in the registers it left from, so no value of the other half is overwritten, and
+// within a half the real parts travel before the imaginary ones.
+__device__ __forceinline__ void q4_exchange_fwd(C2<float>* v, float* sx, int lane) {
+    const int ml = lane >> 2, mb = lane & 3;
+    float* wb = sx + lane;
+    const float* rb = sx + Q4_XROW * ml + mb;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) wb[Q4_XROW * c] = part ? v[q4_slot(c + 16 * s)].y : v[q4_slot(c + 16 * s)].x;
+            wave_sync();
+#pragma unroll
+            for (int src = 0; src < 16; ++src) {
+                const float val = rb[4 * src];
+                if (part) v[q4_slot(src + 16 * s)].y = val;
+                else v[q4_slot(src + 16 * s)].x = val;
+            }
+            wave_sync();
+        }
+}
+
+// the reverse: v[16 s + l] (values of k1 = l' + 16 s from every source l) -> v[k1], natural order
+// (half s returns in the registers it left from)
+__device__ __forceinline__ void q4_exchange_inv(C2<float>* v, float* sx, int lane) {
+    const int ml = lane >> 2, mb = lane & 3;
+    float* wb = sx + Q4_XROW * ml + mb;
+    const float* rb = sx + lane;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+#pragma unroll
+            for (int src = 0; src < 16; ++src) wb[4 * src] = part ? v[16 * s + src].y : v[16 * s + src].x;
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const float val = rb[Q4_XROW * c];
+                if (part) v[16 * s + c].y = val;
+                else v[16 * s + c].x = val;
+            }
+            wave_sync();
+        }
+}
+
+// Pass-2 twiddles T[i][l' + 16 s] = e^{i pi i (4 (l' + 16 s) + 1) / 1024} (i < 16 the source lane),
+// table [l'][s][i] of float2 (4 KB, DevTwiddles::q4tw), read at each use through the cache rather
+// than held in 64 registers.  tt: this lane's row, tt[16 s + i].
+__device__ __forceinline__ C2<float> q4_tw(const C2<float>* __restrict__ tt, int k) {
+    return tt[k];
+}
+
+// an opaque copy of the table row pointer per transform: the loads of a transform cannot be hoisted
+// out of the blind-rotation loop (which would hold 60 registers across it)
+__device__ __forceinline__ const C2<float>* q4_opaque(const C2<float>* p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+// Forward transform, 4 bootstraps per wavefront.  In: v[r] = (d_{16 r + l}, d_{16 r + l + 512}).
+// Out: v[16 s + k2] = F_{l + 16 s + 32 k2}.
+__device__ __forceinline__ void q4_fwd(C2<float>* v, const C2<float>* __restrict__ tt, float* sx, int lane) {
+#pragma unroll
+    for (int r = 1; r < 32; ++r) v[r] = cmul(v[r], C2<float>{kQ4TwRe[r], kQ4TwIm[r]});
+    q4_dft32<+1>(v);
+    q4_exchange_fwd(v, sx, lane);
+    C2<float> h[2][16];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const C2<float>* t = q4_opaque(tt);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) h[s][i] = i ? cmul(v[q4_slot(16 * s + i)], q4_tw(t, 16 * s + i)) : v[q4_slot(16 * s)];
+        dft16<+1>(h[s]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) v[16 * s + k2] = h[s][k2];
+}
+
+// Inverse of q4_fwd without the 1/512 and the uniform untwist: in v[16 s + k2] = F_{l + 16 s + 32 k2};
+// out v[q4_slot(r)] = 512 z_{16 r + l} w^{-16 r} (z_j = p_j + i p_{j+512}).
+__device__ __forceinline__ void q4_inv(C2<float>* v, const C2<float>* __restrict__ tt, float* sx, int lane) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const C2<float>* t = q4_opaque(tt);
+        dft16<-1>(v + 16 * s);
+#pragma unroll
+        for (int i = 1; i < 16; ++i) v[16 * s + i] = cmulc(v[16 * s + i], q4_tw(t, 16 * s + i));
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    q4_exchange_inv(v, sx, lane);
+    q4_dft32<-1>(v);
+}
+
+template <int L, int BG>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_boot32q(BootKArgs A) {
+    constexpr int N = 1024, KPL = 2 * L;
+    static_assert(KPL == 6, "the key row set layout assumes l = 3");
+    constexpr uint32_t OFF = decomp_offset<L, BG>();
+    constexpr int SH = 32 - 11;   // modulus switch to Z_2N
+    __shared__ int32_t s_acc[Q4_NB][2 * N];
+    __shared__ float s_x[Q4_XBUF];
+    const int lane = threadIdx.x, ml = lane >> 2, mb = lane & 3;
+    const int g = blockIdx.x * Q4_NB + mb;
+    const bool active = g < A.L.count;
+    int32_t* acc = s_acc[mb];
+    const int n = A.n;
+
+    // gate inputs of this lane's bootstrap (modes 0 and 3), as boot_prologue picks them
+    const int32_t* xa = nullptr;
+    const int32_t* xb = nullptr;
+    uint32_t al = 0, be = 0;
+    int32_t cst = 0;
+    if (active && (A.L.mode == 0 || A.L.mode == 3)) {
+        if (A.L.mode == 0) {
+            const int s1 = (A.L.nseg > 1 && g >= A.L.seg_start[1]) ? 1 : 0;
+            const int gl = g - A.L.seg_start[s1];
+            xa = A.L.in_a[s1] + (size_t)gl * (n + 1);
+            xb = A.L.in_b[s1] ? A.L.in_b[s1] + (size_t)gl * (n + 1) : nullptr;
+            al = (uint32_t)A.L.alpha[s1];
+            be = (uint32_t)A.L.beta[s1];
+            cst = A.L.cst[s1];
+        } else {
+            const int s = g / A.L.seg_jobs;
+            const int gl = g - s * A.L.seg_jobs;
+            const SegDesc D = A.L.segs[s];
+            xa = D.a + (size_t)gl * (n + 1);
+            xb = D.b ? D.b + (size_t)gl * (n + 1) : nullptr;
+            al = (uint32_t)D.alpha;
+            be = (uint32_t)D.beta;
+            cst = D.cst;
+        }
+    }
+    auto bara_of = [&](int i) -> int {
+        if (!active) return 0;
+        if (A.L.mode == 1) return A.L.bara_in[(size_t)g * n + i];
+        uint32_t x = al * (uint32_t)xa[i];
+        if (xb) x += be * (uint32_t)xb[i];
+        return (int)((x + (1u << (SH - 1))) >> SH);
+    };
+    // prologue: test vector (modes 0, 3) or the accumulator (mode 1), 16 lanes per bootstrap
+    if (A.L.mode == 1) {
+        for (int j = ml; j < 2 * N; j += 16) acc[j] = active ? A.L.acc_io[(size_t)g * 2 * N + j] : 0;
+    } else {
+        int sft = 0;
+        if (active) {
+            uint32_t xbv = (uint32_t)cst + al * (uint32_t)xa[n];
+            if (xb) xbv += be * (uint32_t)xb[n];
+            const int barb = (int)((xbv + (1u << (SH - 1))) >> SH);
+            sft = (2 * N - barb) & (2 * N - 1);
+        }
+        for (int j = ml; j < N; j += 16) {
+            acc[j] = 0;
+            const int e = (j - sft) & (2 * N - 1);
+            acc[N + j] = active ? ((e < N) ? A.L.mu : -A.L.mu) : 0;
+        }
+    }
+    wave_sync();
+
+    const C2<float>* tt = (const C2<float>*)A.t1 + ml * 32;
+    const __amdgpu_buffer_rsrc_t key = key_rsrc(A.bk, (uint32_t)n * Q4_ROWSET_B);
+    const int sb = A.split_bits;
+    int a_next = n > 0 ? bara_of(0) : 0;
+    for (int i = 0; i < n; ++i) {
+        const int a = a_next;
+        if (i + 1 < n) a_next = bara_of(i + 1);
+        q4v4f D[32][2];
+        const uint32_t base4 = rot_base4(ml, a);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            // (X^a - 1) acc_q + offset, xor offset, for j = 16 t + l (t < 64).  The rotation's addresses
+            // and signs are the same for both polynomials; an opaque copy of the base per polynomial
+            // keeps the compiler from computing them once and holding all 64 across the transforms.
+            uint32_t b4 = base4;
+            asm volatile("" : "+v"(b4));
+            uint32_t u[64];
+#pragma unroll
+            for (int t = 0; t < 64; ++t) u[t] = rotdiff_e4<N>(acc + q * N, b4 + 64u * t, 16 * t + ml, OFF) ^ OFF;
+#pragma unroll
+            for (int lvl = 0; lvl < L; ++lvl) {
+                C2<float> v[32];
+#pragma unroll
+                for (int r = 0; r < 32; ++r)
+                    v[r] = C2<float>{(float)digit_sx<BG>(u[r], lvl), (float)digit_sx<BG>(u[r + 32], lvl)};
+                q4_fwd(v, tt, s_x, lane);
+                const uint32_t ro = (uint32_t)i * Q4_ROWSET_B + (uint32_t)(q * L + lvl) * (32u * 1024u);
+#pragma unroll
+                for (int reg = 0; reg < 32; ++reg) {
+                    // at most eight key registers in flight (the scheduler would otherwise issue all
+                    // 32 loads at once: 128 registers)
+                    if (reg % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+                    const float4 kk = key_load(key, (uint32_t)lane * 16u, ro + (uint32_t)reg * 1024u);
+                    if (q == 0 && lvl == 0) {
+                        const q4v4f z = {0.f, 0.f, 0.f, 0.f};
+                        D[reg][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.x, v[reg].x, z, 0, 0, 0);
+                        D[reg][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.z, v[reg].x, z, 0, 0, 0);
+                    } else {
+                        D[reg][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.x, v[reg].x, D[reg][0], 0, 0, 0);
+                        D[reg][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.z, v[reg].x, D[reg][1], 0, 0, 0);
+                    }
+                    D[reg][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.y, v[reg].y, D[reg][0], 0, 0, 0);
+                    D[reg][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(kk.w, v[reg].y, D[reg][1], 0, 0, 0);
+                }
+            }
+        }
+        // inverse transforms; FP64 untwist and rounding; acc += (hi << split) + lo
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            uint32_t rh[64];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                C2<float> v[32];
+#pragma unroll
+                for (int reg = 0; reg < 32; ++reg) v[reg] = C2<float>{D[reg][o][2 * h], D[reg][o][2 * h + 1]};
+                q4_inv(v, tt, s_x, lane);
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    const C2<float> y = v[q4_slot(r)];
+                    const C2<double> z = cmulc(C2<double>{(double)y.x, (double)y.y}, C2<double>{kQ4UntwRe[r], kQ4UntwIm[r]});
+                    if (h == 0) {
+                        rh[2 * r] = round_mod32(z.x);
+                        rh[2 * r + 1] = round_mod32(z.y);
+                    } else {
+                        const int j = o * N + 16 * r + ml;
+                        acc[j] = (int32_t)((uint32_t)acc[j] + (rh[2 * r] << sb) + round_mod32(z.x));
+                        acc[j + 512] = (int32_t)((uint32_t)acc[j + 512] + (rh[2 * r + 1] << sb) + round_mod32(z.y));
+                    }
+                }
+            }
+        }
+        wave_sync();
+    }
+    // epilogue: extraction (modes 0, 3) or the accumulator (mode 1)
+    if (active) {
+        if (A.L.mode == 1) {
+            for (int j = ml; j < 2 * N; j += 16) A.L.acc_io[(size_t)g * 2 * N + j] = acc[j];
+        } else {
+            int32_t* u = A.L.u_out + (size_t)g * (N + 1);
+            for (int j = ml; j < N; j += 16) u[j] = (j == 0) ? acc[0] : (int32_t)(0u - (uint32_t)acc[N - j]);
+            if (ml == 0) u[N] = acc[N];
+        }
+    }
+}
+
+// The key of k_boot32q from the split key bk32 ([n][2l][2][512] float4 {hi.re, hi.im, lo.re,
+// lo.im}, frequency f natural): element (i, row, register 16 s + k2, lane 4 l' + c) holds component c
+// of A_re(K_0), A_im(K_0), A_re(K_1), A_im(K_1) at f = l' + 16 s + 32 k2, where A_re(K) = (hr, hi, lr,
+// li) and A_im(K) = (-hi, hr, -li, lr).  One thread per element.
+__global__ void __launch_bounds__(256) k_keyprep_q4(const float4* __restrict__ bk32, float4* __restrict__ out,
+                                                    size_t total) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int lane = (int)(e & 63), reg = (int)((e >> 6) & 31);
+    const size_t rowi = e >> 11;   // (i * 6 + row)
+    const int c = lane & 3, lp = lane >> 2;
+    const int f = lp + 16 * (reg >> 4) + 32 * (reg & 15);
+    float a[2][2];
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        const float4 K = bk32[(rowi * 2 + o) * 512 + f];
+        const float re[4] = {K.x, K.y, K.z, K.w};
+        const float im[4] = {-K.y, K.x, -K.w, K.z};
+        a[o][0] = re[c];
+        a[o][1] = im[c];
+    }
+    out[e] = make_float4(a[0][0], a[0][1], a[1][0], a[1][1]);
+}
+
+// Transform test of the 4-per-wavefront layout: forward (int32 [count][1024] -> F natural order,
+// interleaved doubles [count][1024]) or inverse (F -> integer-unit doubles [count][1024]).
+__global__ void __launch_bounds__(64) k_fft_test_q4(double* __restrict__ out, const void* __restrict__ in,
+                                                    int inverse, int count, const C2<float>* __restrict__ q4tw) {
+    constexpr int N = 1024, M = 512;
+    __shared__ float s_x[Q4_XBUF];
+    const int lane = threadIdx.x, ml = lane >> 2, mb = lane & 3;
+    const int g = blockIdx.x * Q4_NB + mb;
+    const bool active = g < count;
+    const C2<float>* tt = q4tw + ml * 32;
+    C2<float> v[32];
+    if (!inverse) {
+        const int32_t* p = (const int32_t*)in + (size_t)(active ? g : 0) * N;
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            v[r] = active ? C2<float>{(float)p[16 * r + ml], (float)p[16 * r + ml + M]} : C2<float>{0.f, 0.f};
+        q4_fwd(v, tt, s_x, lane);
+        if (active) {
+            double* o = out + (size_t)g * N;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k2 = 0; k2 < 16; ++k2) {
+                    const int f = ml + 16 * s + 32 * k2;
+                    o[2 * f] = v[16 * s + k2].x;
+                    o[2 * f + 1] = v[16 * s + k2].y;
+                }
+        }
+    } else {
+        const double* F = (const double*)in + (size_t)(active ? g : 0) * N;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k2 = 0; k2 < 16; ++k2) {
+                const int f = ml + 16 * s + 32 * k2;
+                v[16 * s + k2] = active ? C2<float>{(float)F[2 * f], (float)F[2 * f + 1]} : C2<float>{0.f, 0.f};
+            }
+        q4_inv(v, tt, s_x, lane);
+        if (active) {
+            double* o = out + (size_t)g * N;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const C2<float> y = v[q4_slot(r)];
+                const C2<double> z = cmulc(C2<double>{(double)y.x, (double)y.y}, C2<double>{kQ4UntwRe[r], kQ4UntwIm[r]});
+                o[16 * r + ml] = z.x;
+                o[16 * r + ml + M] = z.y;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Fixed scratch for the entry point that allocates nothing (tfhe_gpu_gate_batch_dev_noalloc): the
 // extracted samples between the bootstrap and the keyswitch live in this array instead of a buffer
@@ -2212,6 +2668,7 @@ void set_split_mac(int mode) { g_split_mac = mode; }
 // launch form applied to launches that name none (0: the size rules); see BootLaunch::form
 static int g_forced_form = 0;
 void set_forced_form(int form) { g_forced_form = form; }
+int forced_form() { return g_forced_form; }
 
 // keyswitch fused into the split-FP32 bootstrap kernel (an evaluated alternative; off by default)
 static int g_fused_ks = 0;
@@ -2290,9 +2747,16 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
                 hipLaunchKernelGGL((k_boot32<N, L, BG, 0, true>), dim3(Lc.count), dim3(64), 0, st, A);
                 return hipGetLastError();
             }
-            // forms: 3 the latency form, 2 one bootstrap per wavefront, 0 (and the forms of other
-            // paths) by launch size
+            // forms: 6 four bootstraps per wavefront with the products on the matrix cores, 3 the
+            // latency form, 2 one bootstrap per wavefront, 0 (and the forms of other paths) by
+            // launch size
             int form = Lc.form ? Lc.form : g_forced_form;
+            if (form == 6 && Lc.mode != 2 && key.bk32q && key.p.l == 3 && key.p.n <= kMaxLweN) {
+                A.bk = key.bk32q;
+                A.t1 = tw.q4tw;
+                hipLaunchKernelGGL((k_boot32q<L, BG>), dim3((Lc.count + Q4_NB - 1) / Q4_NB), dim3(64), 0, st, A);
+                return hipGetLastError();
+            }
             if (form != 2 && form != 3) form = 0;
             if (Lc.mode != 2 && (form == 3 || (form == 0 && Lc.count <= boot64_latency_max()))) {
                 hipLaunchKernelGGL((k_boot32lat<N, L, BG>), dim3(Lc.count), dim3(64 * 2 * L), 0, st, A);
@@ -2399,6 +2863,23 @@ hipError_t launch_keyprep(const DevParams& p, const DevTwiddles& tw, const int32
                            (const C2<double>*)tw.rw64);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_keyprep_q4(const DevParams& p, const void* bk32, void* bk32q, hipStream_t st) {
+    if (p.N != 1024 || p.l != 3) return hipErrorInvalidValue;
+    const size_t total = (size_t)p.n * 6 * 32 * 64;
+    hipLaunchKernelGGL(k_keyprep_q4, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, (const float4*)bk32,
+                       (float4*)bk32q, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_test_q4(const DevTwiddles& tw, int inverse, double* out, const void* in, int count,
+                              hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    if (!tw.q4tw) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fft_test_q4, dim3((count + Q4_NB - 1) / Q4_NB), dim3(64), 0, st, out, in, inverse, count,
+                       (const C2<float>*)tw.q4tw);
     return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@ struct tfhe_gpu_key {
     DevParams p{};
     void* bk64 = nullptr;
     void* bk32 = nullptr;
+    void* bk32q = nullptr;     // split key in the layout of launch form 6 (N = 1024)
     int32_t* ksk = nullptr;
     int ksk_np = 0;
     int split_bits = 24;
@@ -40,6 +41,7 @@ struct tfhe_gpu_key {
         v.p = p;
         v.bk64 = bk64;
         v.bk32 = bk32;
+        v.bk32q = bk32q;
         v.split_bits = split_bits;
         v.ksk = ksk;
         v.ksk_np = ksk_np;
@@ -474,7 +476,7 @@ extern "C" int tfhe_gpu_set_fused_keyswitch(int32_t on) {
 }
 
 extern "C" int tfhe_gpu_set_launch_form(int32_t form) {
-    if (form < 0 || form > 5) return arg_fail("launch form must be in [0, 5]");
+    if (form < 0 || form > 6) return arg_fail("launch form must be in [0, 6]");
     std::lock_guard<std::recursive_mutex> g(ctx().mu);
     set_forced_form(form);
     return TFHE_OK;
@@ -539,6 +541,14 @@ extern "C" int tfhe_gpu_key_create(const tfhe_flat_params* fp, const int32_t* bk
         if ((e = hipMalloc(&k->bk32, bk_fft_bytes)) != hipSuccess) return cleanup_fail(hip_fail("hipMalloc bk32", e));
     if ((e = launch_keyprep(p, *tw, d_bk, k->bk64, k->bk32, k->split_bits, c.stream)) != hipSuccess)
         return cleanup_fail(hip_fail("keyprep", e));
+    if (k->bk32 && p.N == 1024 && forced_form() == 6) {
+        // the same split key arranged for the matrix-core form (4 bootstraps per wavefront), made
+        // only for keys created while that form is selected (it is not the default: 124 MB more)
+        if ((e = hipMalloc(&k->bk32q, (size_t)p.n * 6 * 32 * 64 * 16)) != hipSuccess)
+            return cleanup_fail(hip_fail("hipMalloc bk32q", e));
+        if ((e = launch_keyprep_q4(p, k->bk32, k->bk32q, c.stream)) != hipSuccess)
+            return cleanup_fail(hip_fail("keyprep q4", e));
+    }
     if ((e = hipMalloc(&d_ksk_in, ksk_in_bytes)) != hipSuccess) return cleanup_fail(hip_fail("hipMalloc ksk", e));
     if ((e = hipMemcpyAsync(d_ksk_in, ksk, ksk_in_bytes, hipMemcpyHostToDevice, c.stream)) != hipSuccess)
         return cleanup_fail(hip_fail("copy ksk", e));
@@ -597,7 +607,7 @@ extern "C" int tfhe_gpu_key_free(tfhe_gpu_key* k) {
         auto it = c.by_cloud.find(k->owner);
         if (it != c.by_cloud.end() && it->second == k) c.by_cloud.erase(it);
     }
-    void* ps[] = {k->bk64, k->bk32, k->ksk, k->d_u};
+    void* ps[] = {k->bk64, k->bk32, k->bk32q, k->ksk, k->d_u};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (void* p : k->io)
@@ -936,6 +946,24 @@ static int fft_stage(int32_t N, int32_t prec, int inverse, double* out, const vo
     if ((rc = scratch().get(0, ib, &di)) || (rc = scratch().get(1, ob, &dout)) || (rc = h2d(di, in, ib))) return rc;
     hipError_t e = launch_fft_test(*tw, N, (Prec)prec, inverse, (double*)dout, di, count, cx.stream);
     if (e != hipSuccess) return hip_fail("fft launch", e);
+    return d2h_sync(out, dout, ob);
+}
+
+extern "C" int tfhe_gpu_fft_q4_batch(int32_t inverse, double* out, const void* in, int32_t count) {
+    Ctx& cx = ctx();
+    std::lock_guard<std::recursive_mutex> g(cx.mu);
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (count < 0) return arg_fail("negative count");
+    if (count == 0) return TFHE_OK;
+    if (!out || !in) return arg_fail("null argument");
+    DevTwiddles* tw;
+    if ((rc = ensure_twiddles(1024, &tw))) return rc;
+    const size_t ib = (size_t)count * 1024 * (inverse ? 8 : 4), ob = (size_t)count * 1024 * 8;
+    void *di, *dout;
+    if ((rc = scratch().get(0, ib, &di)) || (rc = scratch().get(1, ob, &dout)) || (rc = h2d(di, in, ib))) return rc;
+    hipError_t e = launch_fft_test_q4(*tw, inverse, (double*)dout, di, count, cx.stream);
+    if (e != hipSuccess) return hip_fail("fft q4 launch", e);
     return d2h_sync(out, dout, ob);
 }
 

@@ -123,6 +123,7 @@ def lib():
         "tfhe_gpu_rotate_decompose_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, _i32p, _i32p, i32]),
         "tfhe_gpu_fft_forward_batch": (ctypes.c_int, [i32, i32, _f64p, _i32p, i32]),
         "tfhe_gpu_fft_inverse_batch": (ctypes.c_int, [i32, i32, _f64p, _f64p, i32]),
+        "tfhe_gpu_fft_q4_batch": (ctypes.c_int, [i32, _f64p, _vp, i32]),
         "tfhe_gpu_malloc": (ctypes.c_int, [P(_vp), ctypes.c_size_t]),
         "tfhe_gpu_free": (ctypes.c_int, [_vp]),
         "tfhe_gpu_memcpy_h2d": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
@@ -249,7 +250,8 @@ def set_split_mac(mode):
 
 def set_launch_form(form):
     """Bootstrap launch form for launches that name none (0: size rules; 1 FP64 key sharing at
-    N=1024; 2 one bootstrap per workgroup; 3 latency form; 4, 5 N=2048 with 2 / 4 per workgroup)."""
+    N=1024; 2 one bootstrap per workgroup; 3 latency form; 4, 5 N=2048 with 2 / 4 per workgroup;
+    6 split FP32 at N=1024 with 4 bootstraps per wavefront and the products on the matrix cores)."""
     _ck(lib().tfhe_gpu_set_launch_form(int(form)), "set_launch_form")
 
 
@@ -424,6 +426,25 @@ def fft_inverse(F, prec=PREC_FP64):
     inp = np.ascontiguousarray(np.stack([F.real, F.imag], axis=-1).reshape(cnt, 2 * M))
     out = np.zeros((cnt, 2 * M), np.float64)
     _ck(lib().tfhe_gpu_fft_inverse_batch(2 * M, prec, f64p(out), f64p(inp), cnt), "fft_inverse")
+    return out
+
+
+def fft_q4_forward(polys):
+    """Forward transform (N=1024, FP32) in launch form 6's layout: int32 [count][1024] -> complex F."""
+    polys = np.ascontiguousarray(polys, np.int32)
+    cnt = polys.shape[0]
+    out = np.zeros((cnt, 1024), np.float64)
+    _ck(lib().tfhe_gpu_fft_q4_batch(0, f64p(out), polys.ctypes.data_as(_vp), cnt), "fft_q4_forward")
+    return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+def fft_q4_inverse(F):
+    """Inverse of fft_q4_forward to integer-unit doubles [count][1024]."""
+    F = np.asarray(F, np.complex128)
+    cnt = F.shape[0]
+    inp = np.ascontiguousarray(np.stack([F.real, F.imag], axis=-1).reshape(cnt, 1024))
+    out = np.zeros((cnt, 1024), np.float64)
+    _ck(lib().tfhe_gpu_fft_q4_batch(1, f64p(out), inp.ctypes.data_as(_vp), cnt), "fft_q4_inverse")
     return out
 
 

@@ -354,7 +354,10 @@ int tfhe_gpu_set_split_mac(int32_t mode);
  *   1  FP64 at N = 1024: workgroups of 8 bootstraps sharing each key row through LDS;
  *   2  one bootstrap per workgroup: one wavefront (N = 1024) or a wavefront pair (N = 2048);
  *   3  latency form: one bootstrap spread over a workgroup of several wavefronts;
- *   4, 5  N = 2048: 2 or 4 two-wavefront bootstraps per workgroup.
+ *   4, 5  N = 2048: 2 or 4 two-wavefront bootstraps per workgroup;
+ *   6  split FP32 at N = 1024: 4 bootstraps per wavefront (16 lanes each) with the pointwise
+ *      products on the matrix cores, for keys created while form 6 was selected (their key also
+ *      holds the 124 MB layout this form reads); other keys take the default split form.
  * A form that does not apply to a launch's path is treated as 0.  FP64 results are identical in
  * every form; split-FP32 results differ only in FP32 summation order. */
 int tfhe_gpu_set_launch_form(int32_t form);
@@ -443,6 +446,10 @@ int tfhe_gpu_rotate_decompose_batch(const tfhe_flat_params* p, int32_t* rot, int
  * integer-unit doubles [count][N]. */
 int tfhe_gpu_fft_forward_batch(int32_t N, int32_t prec, double* out, const int32_t* in, int32_t count);
 int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, const double* in, int32_t count);
+/* The same two transforms (N = 1024, FP32) in the layout of launch form 6, four polynomials per
+ * wavefront with 16 lanes each: forward int32 [count][1024] -> F natural order (interleaved
+ * doubles [count][1024]) when inverse = 0, else F -> integer-unit doubles [count][1024]. */
+int tfhe_gpu_fft_q4_batch(int32_t inverse, double* out, const void* in, int32_t count);
 
 /* device memory helpers for the *_dev entry points */
 int tfhe_gpu_malloc(void** ptr, size_t bytes);
