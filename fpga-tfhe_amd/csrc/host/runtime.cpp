@@ -73,7 +73,7 @@ struct Ctx {
 };
 
 Ctx& ctx() {
-    static Ctx* c = new Ctx();  // never destroyed: device memory is released by the driver at exit
+    static Ctx* c = new Ctx();  // never destroyed: device memory is released when the process exits
     return *c;
 }
 
