@@ -66,7 +66,7 @@ def test_forms_n1024(gpu_env):
         assert np.array_equal(f64[1][g], O.blind_rotate(keys, acc[g], bara[g])), g
     s32 = rotate_forms(dk, acc, bara, 1, [2, 3])
     for f, res in s32.items():
-        e = phase_errors(P1, keys, acc, bara, res, (0, 1, cnt - 1))
+        e = phase_errors(P1, keys, acc, bara, res, range(cnt))
         print(f"N=1024 split form {f}: phase error rms {np.sqrt((e**2).mean()):.3e} max {e.max():.3e}")
         assert e.max() < 2**26
 
@@ -85,7 +85,7 @@ def test_forms_n2048(env2048):
         assert np.array_equal(f64[2][g], O.blind_rotate(keys, acc[g], bara[g])), g
     s32 = rotate_forms(dk, acc, bara, 1, [2, 3, 4, 5])
     for f, res in s32.items():
-        e = phase_errors(P2, keys, acc, bara, res, (0, 1, cnt - 1))
+        e = phase_errors(P2, keys, acc, bara, res, range(cnt))
         print(f"N=2048 split form {f}: phase error rms {np.sqrt((e**2).mean()):.3e} max {e.max():.3e}")
         assert np.sqrt((e ** 2).mean()) < 2**25
         assert e.max() < 2**27

@@ -130,16 +130,16 @@ def test_gate_n2048_fp64_bit_exact_vs_oracle(env2048, gate):
 @pytest.mark.parametrize("cnt", [2, 300])
 def test_blind_rotate_n2048_fp64_forms(env2048, cnt):
     """Launches of at most one bootstrap per CU run the latency form (one workgroup per bootstrap,
-    a wavefront pair per level), larger ones the batched two-wavefront kernel.  Accumulators 0, 1
-    and the last are bit-exact with the oracle; on the small launch the latency form and the
-    batched kernel (forced through the FP64 form switch) agree bit for bit."""
+    a wavefront pair per level), larger ones the batched two-wavefront kernel.  Accumulators 0, 1,
+    the last and five spread between them are bit-exact with the oracle; on the small launch the
+    latency form and the batched kernel (forced through the FP64 form switch) agree bit for bit."""
     _, dk, keys = env2048
     rng = np.random.default_rng(2055 + cnt)
     acc = np.stack([O.test_vector(P, int(b), T().MU) for b in rng.integers(0, 2 * N, cnt)])
     bara = rng.integers(0, 2 * N, (cnt, P.n)).astype(np.int32)
     bara[0, ::5] = 0
     got = dk.blind_rotate(acc, bara, prec=0)
-    for g in sorted({0, 1, cnt - 1}):
+    for g in sorted({0, 1, cnt - 1} | set(np.linspace(0, cnt - 1, 7).astype(int).tolist())):
         assert np.array_equal(got[g], O.blind_rotate(keys, acc[g], bara[g])), g
     if cnt <= 256:
         T().set_fp64_shared_rows(True)
@@ -153,7 +153,8 @@ def test_blind_rotate_n2048_fp64_forms(env2048, cnt):
 @pytest.mark.parametrize("cnt", [2, 300])
 def test_blind_rotate_n2048_split32_phase_error(env2048, cnt):
     """Split FP32 at N=2048, latency form (2 accumulators) and batched kernel (300): the TLWE phase
-    of accumulators 0, 1 and the last against the noiseless rotation, beside the FP64 path's.  At
+    of 24 accumulators spread over the launch (both ends included) against the noiseless rotation,
+    beside the FP64 path's.  At
     this set the key noise is small (sigma 2^-30: FP64 rms about 2^20 units) and the split path's
     own error dominates, about 2^24.5 units rms after the 750 steps."""
     _, dk, keys = env2048
@@ -162,9 +163,10 @@ def test_blind_rotate_n2048_split32_phase_error(env2048, cnt):
     bara = rng.integers(0, 2 * N, (cnt, P.n)).astype(np.int32)
     res = {p: dk.blind_rotate(acc, bara, prec=p) for p in (0, 1)}
     rms, worst, mean = {}, {}, {}
+    checked = sorted({0, 1, cnt - 1} | set(np.linspace(0, cnt - 1, 24).astype(int).tolist()))
     for p in (0, 1):
         errs = []
-        for g in sorted({0, 1, cnt - 1}):
+        for g in checked:
             rot = int((bara[g].astype(np.int64) * keys.lwe_key).sum() % (2 * N))
             ideal = O.poly_mul_xai(rot, acc[g, 1])
             ph = O.tlwe_phase(P, res[p][g], keys.tlwe_key)

@@ -75,7 +75,7 @@ def test_q4_blind_rotation_phase(q4_env, cnt):
     stats = {}
     for f, out in res.items():
         errs = []
-        for g in sorted({0, 1, cnt - 1}):
+        for g in sorted({0, 1, cnt - 1} | set(np.linspace(0, cnt - 1, 24).astype(int).tolist())):
             rot = int((bara[g].astype(np.int64) * keys.lwe_key).sum() % (2 * N))
             ideal = O.poly_mul_xai(rot, acc[g, 1])
             ph = O.tlwe_phase(P, out[g], keys.tlwe_key)
