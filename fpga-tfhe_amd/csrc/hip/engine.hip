@@ -2554,6 +2554,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
             int32_t* u = A.L.u_out + (size_t)g * (N + 1);
             for (int j = ml; j < N; j += 16) u[j] = (j == 0) ? acc[0] : (int32_t)(0u - (uint32_t)acc[N - j]);
             if (ml == 0) u[N] = acc[N];
+            if (A.L.acc_io)   // the accumulator too, when the launch asks for it (as boot_epilogue)
+                for (int j = ml; j < 2 * N; j += 16) A.L.acc_io[(size_t)g * 2 * N + j] = acc[j];
         }
     }
 }

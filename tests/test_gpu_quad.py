@@ -101,3 +101,25 @@ def test_q4_gates(q4_env, gate, cnt):
         T().set_launch_form(0)
     want = np.array([O.TRUTH[gate](int(a), int(b), int(c)) for a, b, c in bits.T])
     assert np.array_equal(ks.decrypt(out), want)
+
+
+def test_q4_bootstrap_woks_returns_accumulator(q4_env):
+    """A bootstrap without keyswitch that also asks for its accumulators (mode 0 with an accumulator
+    output): under form 6 every returned sample is the extraction of the returned accumulator, and
+    the samples decrypt to NAND of the inputs (7 bootstraps: the last wavefront is partly idle)."""
+    ks, dk, keys = q4_env
+    rng = np.random.default_rng(711)
+    cnt = 7
+    bits = rng.integers(0, 2, (2, cnt)).astype(np.int32)
+    ca, cb = ks.encrypt(bits[0]), ks.encrypt(bits[1])
+    x = np.stack([O.gate_combine(P, "NAND", ca[i], cb[i]) for i in range(cnt)])
+    T().set_launch_form(6)
+    try:
+        print(f"[q4] bootstrap without keyswitch x{cnt}, accumulators returned", flush=True)
+        u, acc = dk.bootstrap_woks(x, prec=T().PREC_SPLIT32, want_acc=True)
+    finally:
+        T().set_launch_form(0)
+    for g in range(cnt):
+        assert np.array_equal(u[g], O.extract(P, acc[g])), g
+    want = 1 - (bits[0] & bits[1])
+    assert np.array_equal(O.boots_decrypt(u, np.asarray(keys.tlwe_key).reshape(-1)), want)
