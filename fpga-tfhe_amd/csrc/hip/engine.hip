@@ -1487,6 +1487,21 @@ __device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, floa
     sink += p[lane * 32] + p[W2_M * 4 + lane * 32];
 }
 
+// The split kernel's accumulator at N = 2048 carries mirrors of its ends in LDS: the W2_MIRROR words
+// below p0 repeat p0's last W2_MIRROR coefficients and the W2_MIRROR words past p1 repeat p1's first,
+// so that the rotation reads every coefficient at a plain offset from a non-negative base
+// (step2w32).  They are written after the prologue and with every update of those coefficients.
+constexpr int W2_MIRROR = 448;
+// Lane L copies k = L, L + 64, ...: sources N - W2_MIRROR + k and N + k are coefficients that lane
+// L itself wrote in the prologue (both offsets are multiples of 64), so program order suffices.
+__device__ __forceinline__ void w2_mirror_fill(int32_t* s_acc, int lane) {
+    static_assert((W2_N - W2_MIRROR) % 64 == 0 && W2_N % 64 == 0, "mirror sources in the copying lane");
+    for (int k = lane; k < W2_MIRROR; k += 64) {
+        s_acc[k - W2_MIRROR] = s_acc[W2_N - W2_MIRROR + k];
+        s_acc[2 * W2_N + k] = s_acc[W2_N + k];
+    }
+}
+
 // Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.  Each wavefront untwists
 // its own inverse outputs in FP64 (rws: register constants with the 1/1024) and the untwisted values
 // are exchanged, so no wavefront untwists its partner's.  sx_own / sx_other: 9 KB per wavefront,
@@ -1510,11 +1525,38 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     // part in every workgroup barrier below
     if (work) {
     uint32_t u[NC];
+    // (X^a - 1) p_q + offset, xor offset, coefficient j = lane + 64 r.  It reads p at
+    // e = (j - a) mod 2N, negated when e >= N.  With d = lane - a and E = (d mod 2N) >> 6, e >= N
+    // exactly when bit 5 of (E + r) mod 64 is set, so bit r of the low word of
+    // rotr64(0xFFFFFFFF00000000, E) is the sign of register r.  Registers r = 8 g + rr read
+    // p[(b_g + 64 rr) mod N], b_g = (d + 512 g) mod N.  p1 is read at b_g + 64 rr < N + 448, which
+    // the mirror past it covers; p0 from the mirror below it at c_g + 64 rr - 448,
+    // c_g = (b_g + 448) mod N, in [-448, N).  Every base register is non-negative (the 448 goes into
+    // the loads' constant offsets) and no read needs wrap arithmetic.
     auto rotate = [&](int q) {
+        const int32_t* pq = s_acc + q * N;
+        if constexpr (ROT) {
+            const int d = lane - a;
+            const uint32_t E = (uint32_t)(d >> 6) & 63u;
+            const uint32_t hw = (E & 32u) ? 0u : 0xFFFFFFFFu;
+            const uint32_t sgn = __builtin_amdgcn_alignbit(hw, ~hw, E & 31u);
+            const int sh = q ? 0 : W2_MIRROR;
+            const int c0 = (d + sh) & (N - 1), c1 = (d + 512 + sh) & (N - 1);
+            const int base[4] = {c0, c1, c0 ^ (N / 2), c1 ^ (N / 2)};
 #pragma unroll
-        for (int r = 0; r < NC; ++r) {
-            const int j = lane + 64 * r;
-            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
+            for (int r = 0; r < NC; ++r) {
+                const uint32_t pj = (uint32_t)pq[lane + 64 * r];
+                const uint32_t sv = (uint32_t)pq[base[r >> 3] + 64 * (r & 7) - sh];
+                const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
+                // +-p_e = (p_e + m) ^ m.  The opaque c keeps the compiler from reassociating the
+                // sum, so that the xor and the add fuse into one v_xad_u32.
+                uint32_t c = OFF - pj;
+                asm("" : "+v"(c));
+                u[r] = (((sv + m) ^ m) + c) ^ OFF;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[lane + 64 * r] + OFF) ^ OFF;
         }
     };
     auto prep = [&](C2<float>* x, int lvl) {
@@ -1616,8 +1658,14 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                     const int j0 = q * N + w2_coef(wave, lane, aa, 0), j1 = q * N + w2_coef(wave, lane, aa, 1);
                     if (raw) { raw[j0] = ph[aa].x * hs + z.x; raw[j1] = ph[aa].y * hs + z.y; }
                     if (ROT) {
-                        s_acc[j0] = (int32_t)((uint32_t)s_acc[j0] + c0);
-                        s_acc[j1] = (int32_t)((uint32_t)s_acc[j1] + c1);
+                        const int32_t n0 = (int32_t)((uint32_t)s_acc[j0] + c0);
+                        const int32_t n1 = (int32_t)((uint32_t)s_acc[j1] + c1);
+                        s_acc[j0] = n0;
+                        s_acc[j1] = n1;
+                        // the mirrors: p0's coefficients from N - 448 up are wavefront 1's imaginary
+                        // parts with aa >= 1; p1's below 448 are wavefront 0's real parts with aa <= 6
+                        if (q == 0 && wave == 1 && aa >= 1) s_acc[j1 - N] = n1;
+                        if (q == 1 && wave == 0 && aa <= 6) s_acc[j0 + N] = n0;
                     } else {
                         s_acc[j0] = (int32_t)c0;
                         s_acc[j1] = (int32_t)c1;
@@ -1731,7 +1779,9 @@ k_boot2w(BootKArgs A) {
     constexpr int N = W2_N, M = W2_M, KPL = 2 * L;
     constexpr int XB = xbuf_elems<W2_H>();
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
-    __shared__ int32_t s_acc_all[GB][2 * N];
+    // the split path's accumulators carry W2_MIRROR-word mirrors at both ends (step2w32)
+    constexpr int AM = P ? W2_MIRROR : 0;
+    __shared__ int32_t s_acc_all[GB][2 * N + 2 * AM];
     // per wavefront C2<double>[XB]: FP64 transform exchanges; split-FP32 transform exchanges
     // (C2<float>[XB]) and its untwisted outputs, rounded to FP32, in two alternating halves
     __shared__ double s_xd[GB][2][2 * XB];
@@ -1742,9 +1792,15 @@ k_boot2w(BootKArgs A) {
     const int b = wid >> 1, wave = wid & 1;
     const int g = blockIdx.x * GB + b;
     const bool active = g < A.L.count;
-    int32_t* s_acc = s_acc_all[b];
+    int32_t* s_acc = s_acc_all[b] + AM;
     int16_t* s_bara = s_bara_all[b];
-    if (active && wave == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if (active && wave == 0) {
+        boot_prologue<N>(A, g, lane, s_acc, s_bara);
+        // no fence: each lane copies only coefficients it wrote itself (w2_mirror_fill), and a
+        // fence here would count as a possible write to global memory, turning the step's uniform
+        // constant loads into vector loads
+        if constexpr (P == 1) w2_mirror_fill(s_acc, lane);
+    }
     if constexpr (P == 0) {
         // lane c of the pass-2 table holds W_64^{c k2}, k2 < 8
         if (threadIdx.x < 64) s_t2[(threadIdx.x >> 3) * T2S + (threadIdx.x & 7)] = ((const C2<double>*)A.t2)[threadIdx.x];
