@@ -1487,20 +1487,78 @@ __device__ __forceinline__ void w2_warm_row(const void* row_half, int lane, floa
     sink += p[lane * 32] + p[W2_M * 4 + lane * 32];
 }
 
-// The split kernel's accumulator at N = 2048 carries mirrors of its ends in LDS: the W2_MIRROR words
-// below p0 repeat p0's last W2_MIRROR coefficients and the W2_MIRROR words past p1 repeat p1's first,
-// so that the rotation reads every coefficient at a plain offset from a non-negative base
-// (step2w32).  They are written after the prologue and with every update of those coefficients.
-constexpr int W2_MIRROR = 448;
-// Lane L copies k = L, L + 64, ...: sources N - W2_MIRROR + k and N + k are coefficients that lane
-// L itself wrote in the prologue (both offsets are multiples of 64), so program order suffices.
+// The accumulators of the N = 2048 kernels carry mirrors of their ends in LDS: the MW words below
+// p0 repeat p0's last MW coefficients and the MW words past p1 repeat p1's first, so that the
+// rotation reads every coefficient at a plain offset from a non-negative base (w2_rotate).  The
+// rotation reads registers in groups of G, so MW = 64 (G - 1): the split kernel uses G = 8 (fewer
+// base registers), the FP64 kernel G = 4 (its workgroup has LDS for 192-word mirrors only).
+template <int G>
+__host__ __device__ constexpr int w2_mirror() { return 64 * (G - 1); }
+
+// Lane L copies k = L, L + 64, ...: sources N - MW + k and N + k are coefficients that lane L
+// itself wrote in the prologue (both offsets are multiples of 64), so program order suffices.
+template <int G>
 __device__ __forceinline__ void w2_mirror_fill(int32_t* s_acc, int lane) {
-    static_assert((W2_N - W2_MIRROR) % 64 == 0 && W2_N % 64 == 0, "mirror sources in the copying lane");
-    for (int k = lane; k < W2_MIRROR; k += 64) {
-        s_acc[k - W2_MIRROR] = s_acc[W2_N - W2_MIRROR + k];
+    constexpr int MW = w2_mirror<G>();
+    static_assert((W2_N - MW) % 64 == 0 && W2_N % 64 == 0, "mirror sources in the copying lane");
+    for (int k = lane; k < MW; k += 64) {
+        s_acc[k - MW] = s_acc[W2_N - MW + k];
         s_acc[2 * W2_N + k] = s_acc[W2_N + k];
     }
 }
+
+// After wavefront `wave` has written n0 = p_q[w2_coef(wave, lane, aa, 0)] and n1 (its imaginary
+// partner): p0's coefficients from N - MW up are wavefront 1's imaginary parts with
+// aa >= 8 - MW / 64, and p1's below MW are wavefront 0's real parts with aa < MW / 64.
+template <int G>
+__device__ __forceinline__ void w2_mirror_update(int32_t* s_acc, int q, int wave, int aa, int j0, int j1, int32_t n0,
+                                                 int32_t n1) {
+    constexpr int MW = w2_mirror<G>();
+    if (q == 0 && wave == 1 && aa >= 8 - MW / 64) s_acc[j1 - W2_N] = n1;
+    if (q == 1 && wave == 0 && aa < MW / 64) s_acc[j0 + W2_N] = n0;
+}
+
+// u[r] = ((X^a - 1) p_q + offset) ^ offset at coefficient j = lane + 64 r (r < 32), or
+// (p_q + offset) ^ offset when !ROT.  The rotation reads p at e = (j - a) mod 2N, negated when
+// e >= N.  With d = lane - a and E = (d mod 2N) >> 6, e >= N exactly when bit 5 of (E + r) mod 64
+// is set, so bit r of the low word of rotr64(0xFFFFFFFF00000000, E) is the sign of register r (one
+// v_alignbit).  Registers r = G g + rr read p[(b_g + 64 rr) mod N], b_g = (d + 64 G g) mod N.  p1
+// is read at b_g + 64 rr < N + MW, which the mirror past it covers; p0 from the mirror below it at
+// c_g + 64 rr - MW, c_g = (b_g + MW) mod N, in [-MW, N).  Every base register is non-negative (the
+// MW goes into the loads' constant offsets: a negative base register with a positive offset gave
+// wrong results) and no read needs wrap arithmetic.  +-p_e = (p_e + m) ^ m for m = 0 or -1, and the
+// opaque offset term keeps the compiler from reassociating the sum, so that the xor and the add
+// fuse into one v_xad_u32: four instructions per coefficient and one sign extract per register.
+template <int G, bool ROT, uint32_t OFF>
+__device__ __forceinline__ void w2_rotate(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
+    constexpr int N = W2_N, NC = 32, MW = w2_mirror<G>(), NG = NC / G;
+    const int32_t* pq = s_acc + q * N;
+    if constexpr (ROT) {
+        const int d = lane - a;
+        const uint32_t E = (uint32_t)(d >> 6) & 63u;
+        const uint32_t hw = (E & 32u) ? 0u : 0xFFFFFFFFu;
+        const uint32_t sgn = __builtin_amdgcn_alignbit(hw, ~hw, E & 31u);
+        const int sh = q ? 0 : MW;
+        int base[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) base[g] = (d + 64 * G * g + sh) & (N - 1);
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const uint32_t pj = (uint32_t)pq[lane + 64 * r];
+            const uint32_t sv = (uint32_t)pq[base[r / G] + 64 * (r % G) - sh];
+            const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
+            uint32_t c = OFF - pj;
+            asm("" : "+v"(c));
+            u[r] = (((sv + m) ^ m) + c) ^ OFF;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[lane + 64 * r] + OFF) ^ OFF;
+    }
+}
+
+// mirror group sizes of the two N = 2048 kernels (see w2_mirror)
+constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4;
 
 // Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.  Each wavefront untwists
 // its own inverse outputs in FP64 (rws: register constants with the 1/1024) and the untwisted values
@@ -1525,40 +1583,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     // part in every workgroup barrier below
     if (work) {
     uint32_t u[NC];
-    // (X^a - 1) p_q + offset, xor offset, coefficient j = lane + 64 r.  It reads p at
-    // e = (j - a) mod 2N, negated when e >= N.  With d = lane - a and E = (d mod 2N) >> 6, e >= N
-    // exactly when bit 5 of (E + r) mod 64 is set, so bit r of the low word of
-    // rotr64(0xFFFFFFFF00000000, E) is the sign of register r.  Registers r = 8 g + rr read
-    // p[(b_g + 64 rr) mod N], b_g = (d + 512 g) mod N.  p1 is read at b_g + 64 rr < N + 448, which
-    // the mirror past it covers; p0 from the mirror below it at c_g + 64 rr - 448,
-    // c_g = (b_g + 448) mod N, in [-448, N).  Every base register is non-negative (the 448 goes into
-    // the loads' constant offsets) and no read needs wrap arithmetic.
-    auto rotate = [&](int q) {
-        const int32_t* pq = s_acc + q * N;
-        if constexpr (ROT) {
-            const int d = lane - a;
-            const uint32_t E = (uint32_t)(d >> 6) & 63u;
-            const uint32_t hw = (E & 32u) ? 0u : 0xFFFFFFFFu;
-            const uint32_t sgn = __builtin_amdgcn_alignbit(hw, ~hw, E & 31u);
-            const int sh = q ? 0 : W2_MIRROR;
-            const int c0 = (d + sh) & (N - 1), c1 = (d + 512 + sh) & (N - 1);
-            const int base[4] = {c0, c1, c0 ^ (N / 2), c1 ^ (N / 2)};
-#pragma unroll
-            for (int r = 0; r < NC; ++r) {
-                const uint32_t pj = (uint32_t)pq[lane + 64 * r];
-                const uint32_t sv = (uint32_t)pq[base[r >> 3] + 64 * (r & 7) - sh];
-                const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
-                // +-p_e = (p_e + m) ^ m.  The opaque c keeps the compiler from reassociating the
-                // sum, so that the xor and the add fuse into one v_xad_u32.
-                uint32_t c = OFF - pj;
-                asm("" : "+v"(c));
-                u[r] = (((sv + m) ^ m) + c) ^ OFF;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[lane + 64 * r] + OFF) ^ OFF;
-        }
-    };
+    auto rotate = [&](int q) { w2_rotate<W2_G_SPLIT, ROT, OFF>(u, s_acc, q, a, lane); };
     auto prep = [&](C2<float>* x, int lvl) {
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa)
@@ -1662,10 +1687,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                         const int32_t n1 = (int32_t)((uint32_t)s_acc[j1] + c1);
                         s_acc[j0] = n0;
                         s_acc[j1] = n1;
-                        // the mirrors: p0's coefficients from N - 448 up are wavefront 1's imaginary
-                        // parts with aa >= 1; p1's below 448 are wavefront 0's real parts with aa <= 6
-                        if (q == 0 && wave == 1 && aa >= 1) s_acc[j1 - N] = n1;
-                        if (q == 1 && wave == 0 && aa <= 6) s_acc[j0 + N] = n0;
+                        w2_mirror_update<W2_G_SPLIT>(s_acc, q, wave, aa, j0, j1, n0, n1);
                     } else {
                         s_acc[j0] = (int32_t)c0;
                         s_acc[j1] = (int32_t)c1;
@@ -1703,11 +1725,7 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
-#pragma unroll
-        for (int r = 0; r < NC; ++r) {
-            const int j = lane + 64 * r;
-            u[r] = ((ROT ? rotdiff<N>(s_acc + q * N, j, a) : (uint32_t)s_acc[q * N + j]) + OFF) ^ OFF;
-        }
+        w2_rotate<W2_G_FP64, ROT, OFF>(u, s_acc, q, a, lane);
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
             const int row = q * L + lvl;
@@ -1757,8 +1775,11 @@ __device__ __forceinline__ void step2w64(int32_t* s_acc, C2<double>* sx_own, con
             if (raw) { raw[j0] = z.x; raw[j1] = z.y; }
             const uint32_t c0 = round_mod32(z.x), c1 = round_mod32(z.y);
             if (ROT) {
-                s_acc[j0] = (int32_t)((uint32_t)s_acc[j0] + c0);
-                s_acc[j1] = (int32_t)((uint32_t)s_acc[j1] + c1);
+                const int32_t n0 = (int32_t)((uint32_t)s_acc[j0] + c0);
+                const int32_t n1 = (int32_t)((uint32_t)s_acc[j1] + c1);
+                s_acc[j0] = n0;
+                s_acc[j1] = n1;
+                w2_mirror_update<W2_G_FP64>(s_acc, q, wave, aa, j0, j1, n0, n1);
             } else {
                 s_acc[j0] = (int32_t)c0;
                 s_acc[j1] = (int32_t)c1;
@@ -1779,8 +1800,8 @@ k_boot2w(BootKArgs A) {
     constexpr int N = W2_N, M = W2_M, KPL = 2 * L;
     constexpr int XB = xbuf_elems<W2_H>();
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
-    // the split path's accumulators carry W2_MIRROR-word mirrors at both ends (step2w32)
-    constexpr int AM = P ? W2_MIRROR : 0;
+    // the accumulators carry mirrors of their ends (w2_rotate)
+    constexpr int AM = w2_mirror<P ? W2_G_SPLIT : W2_G_FP64>();
     __shared__ int32_t s_acc_all[GB][2 * N + 2 * AM];
     // per wavefront C2<double>[XB]: FP64 transform exchanges; split-FP32 transform exchanges
     // (C2<float>[XB]) and its untwisted outputs, rounded to FP32, in two alternating halves
@@ -1799,7 +1820,7 @@ k_boot2w(BootKArgs A) {
         // no fence: each lane copies only coefficients it wrote itself (w2_mirror_fill), and a
         // fence here would count as a possible write to global memory, turning the step's uniform
         // constant loads into vector loads
-        if constexpr (P == 1) w2_mirror_fill(s_acc, lane);
+        w2_mirror_fill<P ? W2_G_SPLIT : W2_G_FP64>(s_acc, lane);
     }
     if constexpr (P == 0) {
         // lane c of the pass-2 table holds W_64^{c k2}, k2 < 8
