@@ -1908,7 +1908,10 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
     constexpr int T2S = 9;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     using T = typename std::conditional<P == 1, float, double>::type;
-    __shared__ int32_t s_acc[2 * N];
+    // accumulator with mirrored ends (w2_rotate)
+    constexpr int G = W2_G_SPLIT, MW = w2_mirror<G>();
+    __shared__ int32_t s_acc_m[2 * N + 2 * MW];
+    int32_t* const s_acc = s_acc_m + MW;
     __shared__ int16_t s_bara[kMaxLweN];
     __shared__ double s_x[NW][XB];              // transform exchanges: C2<float>[XB] or double[XB] (half width)
     __shared__ C2<T> s_part[2][K][L][W2_H];     // [parity][output][level][frequency]; then the output exchange
@@ -1918,7 +1921,10 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int v = w & 1;
     const int g = blockIdx.x;
-    if (w == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if (w == 0) {
+        boot_prologue<N>(A, g, lane, s_acc, s_bara);
+        w2_mirror_fill<G>(s_acc, lane);   // lane-local copies, no fence (see k_boot2w)
+    }
     if constexpr (P == 0) {
         if (threadIdx.x < 64) s_t2[(threadIdx.x >> 3) * T2S + (threadIdx.x & 7)] = ((const C2<double>*)A.t2)[threadIdx.x];
     }
@@ -1959,11 +1965,7 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                 // 1 is rotated and its digits prepared while row (0, lvl)'s second polynomial is in
                 // flight
                 uint32_t u[NC];
-                auto rotate = [&](int q) {
-#pragma unroll
-                    for (int r = 0; r < NC; ++r)
-                        u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
-                };
+                auto rotate = [&](int q) { w2_rotate<G, true, OFF>(u, s_acc, q, a, lane); };
                 auto prep = [&](C2<float>* x) {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa)
@@ -2011,11 +2013,7 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                 // rotation and digits (all eight fit too, but ran slower), the rest after them
                 constexpr int EK1 = 4;
                 uint32_t u[NC];
-                auto rotate = [&](int q) {
-#pragma unroll
-                    for (int r = 0; r < NC; ++r)
-                        u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
-                };
+                auto rotate = [&](int q) { w2_rotate<G, true, OFF>(u, s_acc, q, a, lane); };
                 auto prep = [&](C2<double>* x) {
 #pragma unroll
                     for (int aa = 0; aa < R1; ++aa) {
@@ -2113,8 +2111,13 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
 #pragma unroll
                 for (int aa = 0; aa < R1; ++aa) {
                     const C2<double> z = w2_pair_own(own[aa], xp[aa * 64 + lane], v);
-                    atomicAdd(accp + w2_coef(v, lane, aa, 0), (int32_t)(round_mod32(z.x) << sh));
-                    atomicAdd(accp + w2_coef(v, lane, aa, 1), (int32_t)(round_mod32(z.y) << sh));
+                    const int32_t d0 = (int32_t)(round_mod32(z.x) << sh), d1 = (int32_t)(round_mod32(z.y) << sh);
+                    const int c0 = w2_coef(v, lane, aa, 0), c1 = w2_coef(v, lane, aa, 1);
+                    atomicAdd(accp + c0, d0);
+                    atomicAdd(accp + c1, d1);
+                    // the mirrors take the same increments (w2_mirror_update's coefficient sets)
+                    if (j == 0 && v == 1 && aa >= 8 - MW / 64) atomicAdd(accp + c1 - N, d1);
+                    if (j == 1 && v == 0 && aa < MW / 64) atomicAdd(accp + c0 + N, d0);
                 }
             } else {
                 int32_t* accp = s_acc + k * N;
@@ -2123,8 +2126,11 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                     const C2<double> y = xp[aa * 64 + lane];
                     const C2<double> z = w2_pair(own[aa], y, v);
                     const int j0 = w2_coef(v, lane, aa, 0), j1 = w2_coef(v, lane, aa, 1);
-                    accp[j0] = (int32_t)((uint32_t)accp[j0] + round_mod32(z.x));
-                    accp[j1] = (int32_t)((uint32_t)accp[j1] + round_mod32(z.y));
+                    const int32_t n0 = (int32_t)((uint32_t)accp[j0] + round_mod32(z.x));
+                    const int32_t n1 = (int32_t)((uint32_t)accp[j1] + round_mod32(z.y));
+                    accp[j0] = n0;
+                    accp[j1] = n1;
+                    w2_mirror_update<G>(s_acc, k, v, aa, k * N + j0, k * N + j1, n0, n1);
                 }
             }
         }
