@@ -292,6 +292,43 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
 }
 
 // ---------------------------------------------------------------------------------------------
+// Rotation of the N = 1024 split latency form from an accumulator with mirrored ends, as w2_rotate does
+// at N = 2048: the 448 words below p0 repeat p0's last 448 coefficients and the 448 past p1 repeat
+// p1's first.  Registers r = 8 g + rr (r < 16) read p[(b_g + 64 rr) mod N], b_0 = d mod N,
+// b_1 = b_0 ^ N/2, d = lane - a: p1 directly (up to N + 447), p0 from c_g = (b_g + 448) mod N at
+// c_g + 64 rr - 448 in [-448, N), so every base register is non-negative.  The sign of register r
+// is bit r of rotr32(0xFFFF0000, E), E = (d mod 2N) >> 6.  u[r] = ((X^a - 1) p_q + offset) ^ offset.
+// ---------------------------------------------------------------------------------------------
+constexpr int kLatMirror = 448;
+__device__ __forceinline__ void lat_mirror_fill(int32_t* s_acc, int lane) {
+    constexpr int N = 1024;
+    static_assert((N - kLatMirror) % 64 == 0, "mirror sources in the copying lane");
+    // lane L copies coefficients it wrote itself in the prologue, so no fence is needed
+    for (int k = lane; k < kLatMirror; k += 64) {
+        s_acc[k - kLatMirror] = s_acc[N - kLatMirror + k];
+        s_acc[2 * N + k] = s_acc[N + k];
+    }
+}
+template <uint32_t OFF>
+__device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
+    constexpr int N = 1024, NC = 16;
+    const int32_t* pq = s_acc + q * N;
+    const int d = lane - a;
+    const uint32_t sgn = __builtin_amdgcn_alignbit(0xFFFF0000u, 0xFFFF0000u, (uint32_t)(d >> 6) & 31u);
+    const int sh = q ? 0 : kLatMirror;
+    const int b0 = (d + sh) & (N - 1), b1 = b0 ^ (N / 2);
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        const uint32_t pj = (uint32_t)pq[lane + 64 * r];
+        const uint32_t sv = (uint32_t)pq[(r < 8 ? b0 : b1) + 64 * (r % 8) - sh];
+        const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
+        uint32_t c = OFF - pj;
+        asm("" : "+v"(c));
+        u[r] = (((sv + m) ^ m) + c) ^ OFF;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // FP64 path, latency form: one bootstrap per workgroup of 2l wavefronts.  In each step wavefront
 // w = (q, lvl) rotates polynomial q of the accumulator, transforms its level-lvl digits and
 // multiplies them by key row w, leaving the two partial products in LDS.  After a barrier,
@@ -339,6 +376,7 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot64lat(BootKArgs A) {
         if (a == 0) continue;
         if (loaded != i) load_row(i);
         {
+            // the byte-offset rotation: the mirrored one (lat_rotate) spills registers in this kernel
             uint32_t u[NC];
             const uint32_t base4 = rot_base4(lane, a);
 #pragma unroll
@@ -767,14 +805,18 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
     static_assert(M == 512 && KPL >= 4, "the 512-point decimation-in-time transform; four output halves");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     constexpr uint32_t ROW_B = (uint32_t)(2 * M * sizeof(float4)), ROWSET_B = KPL * ROW_B;
-    __shared__ int32_t s_acc[2 * N];
+    __shared__ int32_t s_acc_m[2 * N + 2 * kLatMirror];   // accumulator with mirrored ends (lat_rotate)
+    int32_t* const s_acc = s_acc_m + kLatMirror;
     __shared__ int16_t s_bara[kMaxLweN];
     __shared__ C2<float> s_x[KPL][xbuf_elems<M>()];
     __shared__ C2<float> s_part[KPL][4][M];   // [key row][(output polynomial j, half h) = 2j + h]
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int g = blockIdx.x;
-    if (w == 0) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if (w == 0) {
+        boot_prologue<N>(A, g, lane, s_acc, s_bara);
+        lat_mirror_fill(s_acc, lane);
+    }
     __syncthreads();
 
     DitTw<float> tw;
@@ -803,10 +845,7 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
         if (loaded != i) load_row(i);
         {
             uint32_t u[NC];
-            const uint32_t base4 = rot_base4(lane, a);
-#pragma unroll
-            for (int r = 0; r < NC; ++r)
-                u[r] = rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, lane + 64 * r, OFF) ^ OFF;
+            lat_rotate<OFF>(u, s_acc, q, a, lane);
             C2<float> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
@@ -841,8 +880,12 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
             for (int aa = 0; aa < R1; ++aa) {
                 const int jj = lane + 64 * aa;
                 const C2<double> z = cmulc(C2<double>{(double)o[aa].x, (double)o[aa].y}, untw[uoff + aa]);
-                atomicAdd(acc + jj, (int32_t)(round_mod32(z.x) << sh));
-                atomicAdd(acc + jj + M, (int32_t)(round_mod32(z.y) << sh));
+                const int32_t dl = (int32_t)(round_mod32(z.x) << sh), dh = (int32_t)(round_mod32(z.y) << sh);
+                atomicAdd(acc + jj, dl);
+                atomicAdd(acc + jj + M, dh);
+                // the mirrors take the same increments: p0 from N - 448 up, p1 below 448
+                if (j == 0 && aa >= 1) atomicAdd(acc + jj + M - N, dh);
+                if (j == 1 && aa <= 6) atomicAdd(acc + jj + N, dl);
             }
         }
         __syncthreads();
