@@ -99,9 +99,11 @@ def time_path(T, dk, prec, bufs, batch, steps, warmup):
             "boot_ms_avg": boot_ms / max(boot_n, 1), "ks_ms_avg": ks_ms / max(ks_n, 1)}
 
 
-def audit(T, dk, ks, prec, batch, total, rng):
+def audit(T, dk, ks, prec, batch, total, rng, lwe_n=None):
     """Chained NAND levels: each level's outputs are the next level's inputs (random pairing), so
-    every gate sees bootstrapped-and-keyswitched noise.  Decryption on the GPU, plaintext on host."""
+    every gate sees bootstrapped-and-keyswitched noise.  Decryption on the GPU, plaintext on host.
+    lwe_n: LWE dimension of the key set (the N=1024 set's by default)."""
+    nn = n if lwe_n is None else lwe_n
     bits = rng.integers(0, 2, batch).astype(np.int32)
     cur = T.DeviceBuffer.from_array(ks.encrypt(bits))
     nxt_a = T.DeviceBuffer(cur.nbytes)
@@ -110,7 +112,7 @@ def audit(T, dk, ks, prec, batch, total, rng):
     done = fails = 0
     try:
         while done < total:
-            host = cur.to_array((batch, n + 1))
+            host = cur.to_array((batch, nn + 1))
             perm = rng.permutation(batch)
             T.lib().tfhe_gpu_memcpy_h2d(nxt_b.ptr, np.ascontiguousarray(host[perm]).ctypes.data_as(T._vp),
                                         host.nbytes)
@@ -357,11 +359,11 @@ def adders_report(T, dk, ks, count, reps=2):
     return res
 
 
-def n2048_report(T, gates, steps=2, audit_levels=12):
+def n2048_report(T, gates, steps=2, audit_gates=1_000_000):
     """N=2048 set (n=750, l=3, Bgbit=8, ks t=10): `gates` MUX and `gates` XOR on random bits per
     step (one launch each), both paths; decryption failures counted over every step and over a
-    chained audit of `audit_levels` levels of 16384 NANDs per path, and the split path's
-    external-product error against the FP64 path."""
+    chained audit of at least `audit_gates` NANDs per path (levels of 16384, as at N=1024), and the
+    split path's external-product error against the FP64 path."""
     t = time.perf_counter()
     ks = T.KeySet(2048, seed=(0,))
     dk = T.DeviceKey.from_keyset(ks)
@@ -403,19 +405,13 @@ def n2048_report(T, gates, steps=2, audit_levels=12):
             res[name]["single_gate_correct"] = bool(ks.decrypt(o1)[0] == 1 - (bits[0][0] & bits[1][0]))
             log(f"[bench] N=2048 {name}: {res[name]}")
         # chained audit: levels of 16384 NANDs whose inputs are the previous level's outputs
-        # (bootstrapped noise), every output decrypted
-        for name, prec in (("split32", T.PREC_SPLIT32), ("fp64", T.PREC_FP64)):
-            arng = np.random.default_rng(11)
-            ab = arng.integers(0, 2, 16384).astype(np.int32)
-            act = ks.encrypt(ab)
-            fails = 0
-            for _ in range(audit_levels):
-                perm = arng.permutation(ab.size)
-                act = dk.gate("NAND", act, act[perm], prec=prec)
-                ab = 1 - (ab & ab[perm])
-                fails += int((ks.decrypt(act) != ab).sum())
-            res[name]["audit"] = {"gates": audit_levels * ab.size, "failures": fails, "chained": True}
-            log(f"[bench] N=2048 audit {name}: {res[name]['audit']}")
+        # (bootstrapped noise), every output decrypted on the GPU
+        if audit_gates > 0:
+            for name, prec in (("split32", T.PREC_SPLIT32), ("fp64", T.PREC_FP64)):
+                t = time.perf_counter()
+                done, fails = audit(T, dk, ks, prec, 16384, audit_gates, np.random.default_rng(11), lwe_n=nn)
+                res[name]["audit"] = {"gates": done, "failures": fails, "chained": True}
+                log(f"[bench] N=2048 audit {name}: {res[name]['audit']} in {time.perf_counter() - t:.1f} s")
         acc = rng.integers(-2**31, 2**31, (32, 2, 2048), dtype=np.int64).astype(np.int32)
         rows = rng.integers(0, nn, 32).astype(np.int32)
         r64, raw64 = dk.extern_product(acc, rows, prec=T.PREC_FP64, want_raw=True)
@@ -581,7 +577,8 @@ def main():
     variants = (split_variants_report(T, ks, B, max(2, args.steps // 2), args.audit_gates, np.random.default_rng(4))
                 if args.split_variants else None)
     adders = adders_report(T, dk, ks, args.adders) if args.adders > 0 and args.prec == "both" else None
-    n2048 = n2048_report(T, args.n2048_gates) if args.n2048_gates > 0 and args.prec == "both" else None
+    n2048 = (n2048_report(T, args.n2048_gates, audit_gates=args.audit_gates)
+             if args.n2048_gates > 0 and args.prec == "both" else None)
 
     cpu = None
     if not args.no_cpu:
