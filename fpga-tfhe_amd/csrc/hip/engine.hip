@@ -60,55 +60,74 @@ __device__ __forceinline__ constexpr int log2N() {
     return N == 1024 ? 10 : (N == 2048 ? 11 : (N == 512 ? 9 : 12));
 }
 
-// Gate pre-combination + modulus switch + test vector (mode 0) or accumulator load (modes 1, 2).
+// The gate inputs of bootstrap g in modes 0 and 3: the pre-combination x = cst + alpha a + beta b
+// (b absent when xb is null), per input sample.
+struct BootIn {
+    const int32_t* xa;
+    const int32_t* xb;
+    uint32_t al, be;
+    int32_t cst;
+};
+__device__ __forceinline__ BootIn boot_inputs(const BootKArgs& A, int g) {
+    const int n = A.n;
+    BootIn in;
+    if (A.L.mode == 0) {
+        // both entries are read and one is picked with a mask: a select of the two would be
+        // folded into an indexed load, which copies the argument arrays to scratch
+        const bool s1 = A.L.nseg > 1 && g >= A.L.seg_start[1];
+        const uint32_t m = 0u - (uint32_t)s1;
+        auto pick = [m](int32_t v0, int32_t v1) { return (int32_t)(((uint32_t)v0 & ~m) | ((uint32_t)v1 & m)); };
+        auto pickp = [m](const int32_t* p0, const int32_t* p1) {
+            const uint64_t mm = (uint64_t)(int64_t)(int32_t)m;
+            return (const int32_t*)(((uint64_t)p0 & ~mm) | ((uint64_t)p1 & mm));
+        };
+        const int gl = g - pick(A.L.seg_start[0], A.L.seg_start[1]);
+        const int32_t* ia = pickp(A.L.in_a[0], A.L.in_a[1]);
+        const int32_t* ib = pickp(A.L.in_b[0], A.L.in_b[1]);
+        in.xa = ia + (size_t)gl * (n + 1);
+        in.xb = ib ? ib + (size_t)gl * (n + 1) : nullptr;
+        in.al = (uint32_t)pick(A.L.alpha[0], A.L.alpha[1]);
+        in.be = (uint32_t)pick(A.L.beta[0], A.L.beta[1]);
+        in.cst = pick(A.L.cst[0], A.L.cst[1]);
+    } else {
+        const int s = g / A.L.seg_jobs;
+        const int gl = g - s * A.L.seg_jobs;
+        const SegDesc D = A.L.segs[s];
+        in.xa = D.a + (size_t)gl * (n + 1);
+        in.xb = D.b ? D.b + (size_t)gl * (n + 1) : nullptr;
+        in.al = (uint32_t)D.alpha;
+        in.be = (uint32_t)D.beta;
+        in.cst = D.cst;
+    }
+    return in;
+}
+
 // Modulus switch to Z_2N: ((uint64(x) << 32) + interv/2) / interv with interv = 2^64 / 2N
 // (tfhe_bootstrap_woKS_FFT.cpp:3-10) equals (x + 2^(SH-1)) >> SH on uint32, SH = 32 - log2(2N).
+template <int N>
+__device__ __forceinline__ int mod_switch_2n(uint32_t x) {
+    constexpr int SH = 32 - (log2N<N>() + 1);
+    return (int)((x + (1u << (SH - 1))) >> SH);
+}
+
+// Gate pre-combination + modulus switch + test vector (mode 0) or accumulator load (modes 1, 2).
+// s_bara may be null for a kernel that recomputes the rotations at each step.
 template <int N>
 __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lane, int32_t* s_acc,
                                               int16_t* s_bara) {
     const int n = A.n;
     if (A.L.mode == 0 || A.L.mode == 3) {
-        constexpr int SH = 32 - (log2N<N>() + 1);
-        const int32_t* xa;
-        const int32_t* xb;
-        uint32_t al, be;
-        int32_t cst;
-        if (A.L.mode == 0) {
-            // both entries are read and one is picked with a mask: a select of the two would be
-            // folded into an indexed load, which copies the argument arrays to scratch
-            const bool s1 = A.L.nseg > 1 && g >= A.L.seg_start[1];
-            const uint32_t m = 0u - (uint32_t)s1;
-            auto pick = [m](int32_t v0, int32_t v1) { return (int32_t)(((uint32_t)v0 & ~m) | ((uint32_t)v1 & m)); };
-            auto pickp = [m](const int32_t* p0, const int32_t* p1) {
-                const uint64_t mm = (uint64_t)(int64_t)(int32_t)m;
-                return (const int32_t*)(((uint64_t)p0 & ~mm) | ((uint64_t)p1 & mm));
-            };
-            const int gl = g - pick(A.L.seg_start[0], A.L.seg_start[1]);
-            const int32_t* ia = pickp(A.L.in_a[0], A.L.in_a[1]);
-            const int32_t* ib = pickp(A.L.in_b[0], A.L.in_b[1]);
-            xa = ia + (size_t)gl * (n + 1);
-            xb = ib ? ib + (size_t)gl * (n + 1) : nullptr;
-            al = (uint32_t)pick(A.L.alpha[0], A.L.alpha[1]);
-            be = (uint32_t)pick(A.L.beta[0], A.L.beta[1]);
-            cst = pick(A.L.cst[0], A.L.cst[1]);
-        } else {
-            const int s = g / A.L.seg_jobs;
-            const int gl = g - s * A.L.seg_jobs;
-            const SegDesc D = A.L.segs[s];
-            xa = D.a + (size_t)gl * (n + 1);
-            xb = D.b ? D.b + (size_t)gl * (n + 1) : nullptr;
-            al = (uint32_t)D.alpha;
-            be = (uint32_t)D.beta;
-            cst = D.cst;
+        const BootIn in = boot_inputs(A, g);
+        if (s_bara) {
+            for (int i = lane; i < n; i += 64) {
+                uint32_t x = in.al * (uint32_t)in.xa[i];
+                if (in.xb) x += in.be * (uint32_t)in.xb[i];
+                s_bara[i] = (int16_t)mod_switch_2n<N>(x);
+            }
         }
-        for (int i = lane; i < n; i += 64) {
-            uint32_t x = al * (uint32_t)xa[i];
-            if (xb) x += be * (uint32_t)xb[i];
-            s_bara[i] = (int16_t)((x + (1u << (SH - 1))) >> SH);
-        }
-        uint32_t xbv = (uint32_t)cst + al * (uint32_t)xa[n];
-        if (xb) xbv += be * (uint32_t)xb[n];
-        const int barb = (int)((xbv + (1u << (SH - 1))) >> SH);
+        uint32_t xbv = (uint32_t)in.cst + in.al * (uint32_t)in.xa[n];
+        if (in.xb) xbv += in.be * (uint32_t)in.xb[n];
+        const int barb = mod_switch_2n<N>(xbv);
         // b = X^{2N - barb} [mu, ..., mu] (mulByXaiOrCopy.cpp:25-41), a = 0 (tb :317)
         const int sft = (2 * N - barb) & (2 * N - 1);
         for (int j = lane; j < N; j += 64) {
@@ -119,7 +138,7 @@ __device__ __forceinline__ void boot_prologue(const BootKArgs& A, int g, int lan
     } else {
         const int32_t* ain = A.L.acc_io + (size_t)g * 2 * N;
         for (int j = lane; j < 2 * N; j += 64) s_acc[j] = ain[j];
-        if (A.L.mode == 1)
+        if (A.L.mode == 1 && s_bara)
             for (int i = lane; i < n; i += 64) s_bara[i] = (int16_t)A.L.bara_in[(size_t)g * n + i];
     }
 }
@@ -292,40 +311,54 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
 }
 
 // ---------------------------------------------------------------------------------------------
-// Rotation of the N = 1024 split latency form from an accumulator with mirrored ends, as w2_rotate does
-// at N = 2048: the 448 words below p0 repeat p0's last 448 coefficients and the 448 past p1 repeat
-// p1's first.  Registers r = 8 g + rr (r < 16) read p[(b_g + 64 rr) mod N], b_0 = d mod N,
-// b_1 = b_0 ^ N/2, d = lane - a: p1 directly (up to N + 447), p0 from c_g = (b_g + 448) mod N at
-// c_g + 64 rr - 448 in [-448, N), so every base register is non-negative.  The sign of register r
-// is bit r of rotr32(0xFFFF0000, E), E = (d mod 2N) >> 6.  u[r] = ((X^a - 1) p_q + offset) ^ offset.
+// Rotation at N = 1024 from an accumulator with mirrored ends, as w2_rotate does at N = 2048: the
+// MW = 64 (G - 1) words below p0 repeat p0's last MW coefficients and the MW past p1 repeat p1's
+// first.  Registers r = G g + rr (r < 16) read p[(b_g + 64 rr) mod N], b_g = (d + 64 G g) mod N,
+// d = lane - a: p1 directly (up to N + MW - 1), p0 from c_g = (b_g + MW) mod N at c_g + 64 rr - MW
+// in [-MW, N), so every base register is non-negative.  The sign of register r is bit r of
+// rotr32(0xFFFF0000, E), E = (d mod 2N) >> 6.  u[r] = ((X^a - 1) p_q + offset) ^ offset.  Used by
+// the split latency form (G = 8) and the FP64 key-sharing form (G = 4, what its LDS has room for).
 // ---------------------------------------------------------------------------------------------
-constexpr int kLatMirror = 448;
+template <int G>
+__host__ __device__ constexpr int lat_mirror() { return 64 * (G - 1); }
+template <int G>
 __device__ __forceinline__ void lat_mirror_fill(int32_t* s_acc, int lane) {
-    constexpr int N = 1024;
-    static_assert((N - kLatMirror) % 64 == 0, "mirror sources in the copying lane");
+    constexpr int N = 1024, MW = lat_mirror<G>();
+    static_assert((N - MW) % 64 == 0, "mirror sources in the copying lane");
     // lane L copies coefficients it wrote itself in the prologue, so no fence is needed
-    for (int k = lane; k < kLatMirror; k += 64) {
-        s_acc[k - kLatMirror] = s_acc[N - kLatMirror + k];
+    for (int k = lane; k < MW; k += 64) {
+        s_acc[k - MW] = s_acc[N - MW + k];
         s_acc[2 * N + k] = s_acc[N + k];
     }
 }
-template <uint32_t OFF>
+template <int G, uint32_t OFF>
 __device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
-    constexpr int N = 1024, NC = 16;
+    constexpr int N = 1024, NC = 16, MW = lat_mirror<G>(), NG = NC / G;
     const int32_t* pq = s_acc + q * N;
     const int d = lane - a;
     const uint32_t sgn = __builtin_amdgcn_alignbit(0xFFFF0000u, 0xFFFF0000u, (uint32_t)(d >> 6) & 31u);
-    const int sh = q ? 0 : kLatMirror;
-    const int b0 = (d + sh) & (N - 1), b1 = b0 ^ (N / 2);
+    const int sh = q ? 0 : MW;
+    int base[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) base[g] = (d + 64 * G * g + sh) & (N - 1);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
         const uint32_t pj = (uint32_t)pq[lane + 64 * r];
-        const uint32_t sv = (uint32_t)pq[(r < 8 ? b0 : b1) + 64 * (r % 8) - sh];
+        const uint32_t sv = (uint32_t)pq[base[r / G] + 64 * (r % G) - sh];
         const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
         uint32_t c = OFF - pj;
         asm("" : "+v"(c));
         u[r] = (((sv + m) ^ m) + c) ^ OFF;
     }
+}
+// After coefficients j and j + N/2 of p_q (j = lane + 64 aa, aa < 8, the index within p_q) have been
+// updated to lo and hi: p0's coefficients from N - MW up are hi values with aa >= 8 - MW / 64, p1's
+// below MW are lo values with aa < MW / 64.
+template <int G>
+__device__ __forceinline__ void lat_mirror_update(int32_t* s_acc, int q, int aa, int j, int32_t lo, int32_t hi) {
+    constexpr int N = 1024, MW = lat_mirror<G>();
+    if (q == 0 && aa >= 8 - MW / 64) s_acc[j + N / 2 - N] = hi;
+    if (q == 1 && aa < MW / 64) s_acc[2 * N + j] = lo;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -434,7 +467,10 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot64lat(BootKArgs A) {
 // has two transforms' time to land.
 // ---------------------------------------------------------------------------------------------
 constexpr int B64W_GATES = 8;
-constexpr int B64W_MAXN = 640;   // LWE dimension the per-bootstrap bara array holds
+constexpr int B64W_MAXN = 640;   // largest LWE dimension this form is launched for
+#ifndef B64W_MIRROR_G
+#define B64W_MIRROR_G 4   // mirror group size of the accumulators (lat_rotate)
+#endif
 
 template <int M, int W>
 __device__ __forceinline__ void key_row_share(C2<double>* kbuf, const C2<double>* __restrict__ src, int wave,
@@ -475,9 +511,11 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
     constexpr int PER = 2 * R1 / W;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     constexpr size_t ROW = (size_t)2 * M, ROWSET = (size_t)KPL * ROW;
-    __shared__ int32_t s_acc_all[W][2 * N];
+    // accumulators with mirrored ends (lat_rotate); the modulus-switched rotations are not kept in
+    // LDS but recomputed from the inputs 64 steps at a time (one per lane, read with v_readlane)
+    constexpr int G = B64W_MIRROR_G, MW = lat_mirror<G>();
+    __shared__ int32_t s_acc_all[W][2 * N + 2 * MW];
     __shared__ double s_x_all[W][xbuf_elems<M>()];
-    __shared__ int16_t s_bara_all[W][B64W_MAXN];
     __shared__ C2<double> s_k0[2 * M];
     __shared__ C2<double> s_k1[2 * M];
     __shared__ C2<double> s_k2[2 * M];
@@ -485,11 +523,24 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int g = blockIdx.x * W + wave;
     const bool active = g < A.L.count;
-    int32_t* s_acc = s_acc_all[wave];
+    int32_t* s_acc = s_acc_all[wave] + MW;
     double* s_x = s_x_all[wave];
-    int16_t* s_bara = s_bara_all[wave];
-    if (active) boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    if (active) {
+        boot_prologue<N>(A, g, lane, s_acc, nullptr);
+        lat_mirror_fill<G>(s_acc, lane);
+    }
     wave_sync();
+    const int n = A.n;
+    const BootIn in = (active && A.L.mode != 1) ? boot_inputs(A, g) : BootIn{nullptr, nullptr, 0u, 0u, 0};
+    // rotation of step i (0 past the last step, or for an idle wavefront)
+    auto bara_at = [&](int i) -> int {
+        if (!active || i >= n) return 0;
+        if (A.L.mode == 1) return A.L.bara_in[(size_t)g * n + i];
+        uint32_t x = in.al * (uint32_t)in.xa[i];
+        if (in.xb) x += in.be * (uint32_t)in.xb[i];
+        return mod_switch_2n<N>(x);
+    };
+    int bcur = bara_at(lane), bnext = bara_at(64 + lane);   // steps 0-63 and 64-127
 
     LaneTw<double, R1> tw;
     load_lane_tw(tw, (const C2<double>*)A.t1, (const C2<double>*)A.t2, lane);
@@ -497,13 +548,16 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
 #pragma unroll
     for (int r = 0; r < R1; ++r) wa[r] = ((const C2<double>*)A.wa64)[r];
     const C2<double>* key = (const C2<double>*)A.bk;
-    const int n = A.n;
     C2<double>* const kb[3] = {s_k0, s_k1, s_k2};
 
     key_row_share<M, W>(s_k0, key, wave, lane);
     key_row_share<M, W>(s_k1, key + ROW, wave, lane);
     for (int i = 0; i < n; ++i) {
-        const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
+        if ((i & 63) == 0 && i > 0) {
+            bcur = bnext;
+            bnext = bara_at(i + 64 + lane);
+        }
+        const int a = __builtin_amdgcn_readlane(bcur, i & 63);
         // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); an idle wavefront still
         // takes part in the barriers and in loading the shared rows
         const bool work = a != 0;
@@ -514,8 +568,7 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
         for (int q = 0; q < 2; ++q) {
             uint32_t u[NC];
             if (work) {
-#pragma unroll
-                for (int r = 0; r < NC; ++r) u[r] = (rotdiff<N>(s_acc + q * N, lane + 64 * r, a) + OFF) ^ OFF;
+                lat_rotate<G, OFF>(u, s_acc, q, a, lane);
             }
 #pragma unroll
             for (int lvl = 0; lvl < L; ++lvl) {
@@ -558,10 +611,16 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) k_boot64w(BootKArgs A) {
                 const int j = lane + 64 * aa;
                 C2<double> z0 = aa ? cmulc(o0[aa], wa[aa]) : o0[aa];
                 C2<double> z1 = aa ? cmulc(o1[aa], wa[aa]) : o1[aa];
-                s_acc[j] = (int32_t)((uint32_t)s_acc[j] + round_mod32(z0.x * sc));
-                s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + round_mod32(z0.y * sc));
-                s_acc[N + j] = (int32_t)((uint32_t)s_acc[N + j] + round_mod32(z1.x * sc));
-                s_acc[N + j + M] = (int32_t)((uint32_t)s_acc[N + j + M] + round_mod32(z1.y * sc));
+                const int32_t a0 = (int32_t)((uint32_t)s_acc[j] + round_mod32(z0.x * sc));
+                const int32_t a1 = (int32_t)((uint32_t)s_acc[j + M] + round_mod32(z0.y * sc));
+                const int32_t b0 = (int32_t)((uint32_t)s_acc[N + j] + round_mod32(z1.x * sc));
+                const int32_t b1 = (int32_t)((uint32_t)s_acc[N + j + M] + round_mod32(z1.y * sc));
+                s_acc[j] = a0;
+                s_acc[j + M] = a1;
+                s_acc[N + j] = b0;
+                s_acc[N + j + M] = b1;
+                lat_mirror_update<G>(s_acc, 0, aa, j, a0, a1);
+                lat_mirror_update<G>(s_acc, 1, aa, j, b0, b1);
             }
             wave_sync();
         }
@@ -805,8 +864,9 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
     static_assert(M == 512 && KPL >= 4, "the 512-point decimation-in-time transform; four output halves");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     constexpr uint32_t ROW_B = (uint32_t)(2 * M * sizeof(float4)), ROWSET_B = KPL * ROW_B;
-    __shared__ int32_t s_acc_m[2 * N + 2 * kLatMirror];   // accumulator with mirrored ends (lat_rotate)
-    int32_t* const s_acc = s_acc_m + kLatMirror;
+    constexpr int MW = lat_mirror<8>();
+    __shared__ int32_t s_acc_m[2 * N + 2 * MW];   // accumulator with mirrored ends (lat_rotate)
+    int32_t* const s_acc = s_acc_m + MW;
     __shared__ int16_t s_bara[kMaxLweN];
     __shared__ C2<float> s_x[KPL][xbuf_elems<M>()];
     __shared__ C2<float> s_part[KPL][4][M];   // [key row][(output polynomial j, half h) = 2j + h]
@@ -815,7 +875,7 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
     const int g = blockIdx.x;
     if (w == 0) {
         boot_prologue<N>(A, g, lane, s_acc, s_bara);
-        lat_mirror_fill(s_acc, lane);
+        lat_mirror_fill<8>(s_acc, lane);
     }
     __syncthreads();
 
@@ -845,7 +905,7 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
         if (loaded != i) load_row(i);
         {
             uint32_t u[NC];
-            lat_rotate<OFF>(u, s_acc, q, a, lane);
+            lat_rotate<8, OFF>(u, s_acc, q, a, lane);
             C2<float> v[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
