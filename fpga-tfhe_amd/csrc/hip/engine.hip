@@ -967,6 +967,9 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
 // so their reads stay coalesced.
 // ---------------------------------------------------------------------------------------------
 constexpr int KS_GATES = 256;
+#ifndef TFHE_KS_C
+#define TFHE_KS_C 32   // output coefficients per workgroup (registers per lane)
+#endif
 constexpr int KS_IBLK = 16;
 
 template <int TKS, int KN, int NP, int C>
@@ -3012,7 +3015,7 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
 template <int TKS, int KN, int NP>
 static hipError_t launch_ks_t(const DevKeyView& key, int32_t* out, const int32_t* u1, const int32_t* u2,
                               int32_t add_b, int count, hipStream_t st, int32_t* const* out_tab, int out_seg) {
-    constexpr int C = 32;
+    constexpr int C = TFHE_KS_C;
     if (key.ksk_np != NP || key.p.N != KN) return hipErrorInvalidValue;
     if (count <= KS_SMALL_MAX) {
         hipLaunchKernelGGL(k_ks_clear, dim3(count), dim3(256), 0, st, out, key.p.n, out_tab, out_seg);
