@@ -468,6 +468,9 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot64lat(BootKArgs A) {
 // ---------------------------------------------------------------------------------------------
 constexpr int B64W_GATES = 8;
 constexpr int B64W_MAXN = 640;   // largest LWE dimension this form is launched for
+#ifndef K32_MIRROR_G
+#define K32_MIRROR_G 8   // mirror group size of the split kernel's accumulator (lat_rotate)
+#endif
 #ifndef B64W_MIRROR_G
 #define B64W_MIRROR_G 4   // mirror group size of the accumulators (lat_rotate)
 #endif
@@ -672,11 +675,11 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
-        const uint32_t base4 = rot_base4(lane, a);
+        if constexpr (ROT) {
+            lat_rotate<K32_MIRROR_G, OFF>(u, s_acc, q, a, lane);
+        } else {
 #pragma unroll
-        for (int r = 0; r < NC; ++r) {
-            const int j = lane + 64 * r;
-            u[r] = (ROT ? rotdiff_e4<N>(s_acc + q * N, base4 + 256u * r, j, OFF) : (uint32_t)s_acc[q * N + j] + OFF) ^ OFF;
+            for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)s_acc[q * N + lane + 64 * r] + OFF) ^ OFF;
         }
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
@@ -747,10 +750,16 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             raw_hi[N + j] = zh1.x; raw_hi[N + j + M] = zh1.y;
         }
         if (ROT) {
-            s_acc[j] = (int32_t)((uint32_t)s_acc[j] + r00);
-            s_acc[j + M] = (int32_t)((uint32_t)s_acc[j + M] + r01);
-            s_acc[N + j] = (int32_t)((uint32_t)s_acc[N + j] + r10);
-            s_acc[N + j + M] = (int32_t)((uint32_t)s_acc[N + j + M] + r11);
+            const int32_t a0 = (int32_t)((uint32_t)s_acc[j] + r00);
+            const int32_t a1 = (int32_t)((uint32_t)s_acc[j + M] + r01);
+            const int32_t b0 = (int32_t)((uint32_t)s_acc[N + j] + r10);
+            const int32_t b1 = (int32_t)((uint32_t)s_acc[N + j + M] + r11);
+            s_acc[j] = a0;
+            s_acc[j + M] = a1;
+            s_acc[N + j] = b0;
+            s_acc[N + j + M] = b1;
+            lat_mirror_update<K32_MIRROR_G>(s_acc, 0, aa, j, a0, a1);
+            lat_mirror_update<K32_MIRROR_G>(s_acc, 1, aa, j, b0, b1);
         } else {
             s_acc[j] = (int32_t)r00; s_acc[j + M] = (int32_t)r01;
             s_acc[N + j] = (int32_t)r10; s_acc[N + j + M] = (int32_t)r11;
@@ -809,12 +818,16 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1,
                                    N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1))) k_boot32(BootKArgs A) {
     constexpr int M = N / 2, R1 = M / 64, KPL = 2 * L;
-    __shared__ int32_t s_acc[2 * N];
+    // the accumulator with mirrored ends (lat_rotate); the modulus-switched rotations are read 64
+    // steps at a time from the inputs (one per lane, v_readlane per step) instead of from LDS
+    constexpr int MW = lat_mirror<K32_MIRROR_G>();
+    __shared__ int32_t s_acc_m[2 * N + 2 * MW];
     __shared__ C2<float> s_buf[xbuf_elems<M>()];
-    __shared__ int16_t s_bara[kMaxLweN];
+    int32_t* const s_acc = s_acc_m + MW;
     const int lane = threadIdx.x;
     const int g = blockIdx.x;
-    boot_prologue<N>(A, g, lane, s_acc, s_bara);
+    boot_prologue<N>(A, g, lane, s_acc, nullptr);
+    lat_mirror_fill<K32_MIRROR_G>(s_acc, lane);
     wave_sync();
 
     DitTw<float> tw;
@@ -836,8 +849,21 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1
                                      sb, raw, raw_hi);
     } else {
         const int n = A.n;
+        const BootIn in = A.L.mode != 1 ? boot_inputs(A, g) : BootIn{nullptr, nullptr, 0u, 0u, 0};
+        auto bara_at = [&](int i) -> int {   // rotation of step i, 0 past the last step
+            if (i >= n) return 0;
+            if (A.L.mode == 1) return A.L.bara_in[(size_t)g * n + i];
+            uint32_t x = in.al * (uint32_t)in.xa[i];
+            if (in.xb) x += in.be * (uint32_t)in.xb[i];
+            return mod_switch_2n<N>(x);
+        };
+        int bcur = bara_at(lane), bnext = bara_at(64 + lane);
         for (int i = 0; i < n; ++i) {
-            const int a = __builtin_amdgcn_readfirstlane((int)s_bara[i]);
+            if ((i & 63) == 0 && i > 0) {
+                bcur = bnext;
+                bnext = bara_at(i + 64 + lane);
+            }
+            const int a = __builtin_amdgcn_readlane(bcur, i & 63);
             if (a == 0) continue;
             step32<N, L, BG, true, MAC>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb,
                                         nullptr, nullptr);
