@@ -23,6 +23,11 @@ struct DevTwiddles {
     void* twist32 = nullptr;     // [64][R1] C2<float>:  w^{64a + L}      (split path, forward twist)
     void* untw64 = nullptr;      // [64][R1] C2<double>: w^{64a + L} / M  (split path, untwist)
     void* wa64 = nullptr;        // [R1]     C2<double>: w^{64a}
+    // N = 1024 split path, forward transform of digits with its first pass on the matrix cores
+    // (fft512_fwd_mfma): this lane's FP16 pieces of the real 16x16 pass-1 matrix, and the pass-2
+    // input twiddles with the lane twist folded in
+    void* fwd_a16 = nullptr;     // [64][16] FP16: this lane's entries of the two matrix pieces
+    void* fwd_t2 = nullptr;      // [64][8] C2<float>: W_64^{(L>>3) b} w^{8b + (L&7)}
     // N = 2048, one transform over two wavefronts (wavefront v holds the frequencies of parity v):
     void* t1w64 = nullptr;       // [2][64][8] C2<double>: w^{L (1 + 4v + 8 k1)}
     void* t1w32 = nullptr;       // [2][64][8] C2<float>
@@ -147,6 +152,8 @@ hipError_t launch_lwe_linear(const LinDesc* d_descs, int ndesc, int count, int n
 hipError_t launch_extract(int N, int32_t* u, const int32_t* acc, int count, hipStream_t st);
 hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* digits, const int32_t* acc,
                                    const int32_t* bara, int count, hipStream_t st);
+// inverse: 0 forward, 1 inverse, 2 the split path's forward of digit inputs with its first pass on
+// the matrix cores (N = 1024, split FP32 only)
 hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse, double* out, const void* in,
                            int count, hipStream_t st);
 hipError_t launch_valu_probe(int fp64, void* out, int blocks, int iters, hipStream_t st);

@@ -39,6 +39,11 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #ifndef TFHE_BOOT32_WAVES
 #define TFHE_BOOT32_WAVES 2
 #endif
+// Split path at N = 1024: 1 = the forward transforms of the digits take their first pass on the
+// matrix cores (fft512_fwd_mfma), 0 = every pass on the vector ALU (fft512_dit)
+#ifndef TFHE_FWD_MFMA
+#define TFHE_FWD_MFMA 1
+#endif
 
 struct BootKArgs {
     BootLaunch L;
@@ -53,6 +58,8 @@ struct BootKArgs {
     int n;
     int split_bits;
     const int32_t* ksk;     // digit-pair keyswitch key (fused keyswitch form)
+    const void* fwd_a16;    // N = 1024 split path: matrix-core pass-1 pieces (fft512_fwd_mfma)
+    const void* fwd_t2;     // and its pass-2 table with the lane twist folded in
 };
 
 template <int N>
@@ -657,8 +664,8 @@ __device__ __forceinline__ float4 key_load(__amdgpu_buffer_rsrc_t rs, uint32_t l
 template <int N, int L, int BG, bool ROT, int MAC = 0>
 __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, __amdgpu_buffer_rsrc_t kr,
                                        uint32_t row0, const DitTw<float>& tw, const C2<float>* tws,
-                                       const C2<double>* __restrict__ untw, int lane, int split_bits, double* raw,
-                                       double* raw_hi) {
+                                       const FwdMfmaA& am, const C2<double>* __restrict__ untw, int lane,
+                                       int split_bits, double* raw, double* raw_hi) {
     constexpr int M = N / 2, R1 = M / 64, NC = 2 * R1;
     static_assert(M == 512, "the split path runs the 512-point decimation-in-time transform");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
@@ -684,10 +691,19 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
             C2<float> v[R1];
+#if TFHE_FWD_MFMA
+            // tws: the pass-2 table with the lane twist folded in (fft512_fwd_mfma)
+            uint32_t dw[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa)
+                dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
+            fft512_fwd_mfma(v, dw, am, tws, tw, s_buf, lane);
+#else
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
                 v[aa] = C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)};
             fft512_dit<+1, true>(v, tw, tws, s_buf, lane);
+#endif
             const uint32_t ro = row0 + (uint32_t)((q * L + lvl) * 2 * M * sizeof(float4));
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
@@ -832,9 +848,17 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1
 
     DitTw<float> tw;
     load_dit_tw(tw, (const C2<float>*)A.t2, (const C2<float>*)A.t1, lane);
+    // forward twist (fft512_dit), or the matrix-core form's pass-2 table and matrix pieces
     C2<float> tws[R1];
+    FwdMfmaA am;
+#if TFHE_FWD_MFMA
+#pragma unroll
+    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.fwd_t2)[lane * R1 + r];
+    load_fwd_mfma_a(am, (const f16x8_t*)A.fwd_a16, lane);
+#else
 #pragma unroll
     for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
+#endif
     const C2<double>* untw = (const C2<double>*)A.untw64;
     // one row set (2L rows of 2 polynomials of M float4) per LWE coefficient; the whole key is far
     // below 4 GiB, so byte offsets fit 32 bits
@@ -845,7 +869,7 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1
     if (A.L.mode == 2) {
         double* raw = A.L.raw_out ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
         double* raw_hi = A.L.raw_hi ? A.L.raw_hi + (size_t)g * 2 * N : nullptr;
-        step32<N, L, BG, false, MAC>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, untw, lane,
+        step32<N, L, BG, false, MAC>(s_acc, s_buf, 0, key, (uint32_t)A.L.row_set[g] * ROWSET_B, tw, tws, am, untw, lane,
                                      sb, raw, raw_hi);
     } else {
         const int n = A.n;
@@ -865,7 +889,7 @@ __attribute__((amdgpu_waves_per_eu(N <= 1024 && MAC == 0 ? TFHE_BOOT32_WAVES : 1
             }
             const int a = __builtin_amdgcn_readlane(bcur, i & 63);
             if (a == 0) continue;
-            step32<N, L, BG, true, MAC>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, untw, lane, sb,
+            step32<N, L, BG, true, MAC>(s_acc, s_buf, a, key, (uint32_t)i * ROWSET_B, tw, tws, am, untw, lane, sb,
                                         nullptr, nullptr);
         }
     }
@@ -1346,11 +1370,15 @@ __global__ void __launch_bounds__(64) k_fft_test64(double* __restrict__ out, con
     }
 }
 
+// mode 0: forward (fft512_dit with the twist); 1: inverse; 2: forward of digit inputs (|p_j| < 2048)
+// with the first pass on the matrix cores (fft512_fwd_mfma), as the split bootstrap kernel runs it
 template <int N>
 __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, const void* __restrict__ in, int inverse,
                                                    const C2<float>* __restrict__ t1, const C2<float>* __restrict__ t2,
                                                    const C2<float>* __restrict__ twist32,
-                                                   const C2<double>* __restrict__ untw64) {
+                                                   const C2<double>* __restrict__ untw64,
+                                                   const f16x8_t* __restrict__ fwd_a16,
+                                                   const C2<float>* __restrict__ fwd_t2) {
     constexpr int M = N / 2, R1 = M / 64;
     __shared__ C2<float> s_buf[xbuf_elems<M>()];
     const int lane = threadIdx.x;
@@ -1358,14 +1386,26 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
     DitTw<float> tw;
     load_dit_tw(tw, t2, t1, lane);
     C2<float> v[R1], tws[R1];
-    if (!inverse) {
+    if (inverse != 1) {
         const int32_t* p = (const int32_t*)in + g * N;
+        if (inverse == 2) {
+            FwdMfmaA am;
+            load_fwd_mfma_a(am, fwd_a16, lane);
+            uint32_t dw[R1];
 #pragma unroll
-        for (int aa = 0; aa < R1; ++aa) {
-            v[aa] = C2<float>{(float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]};
-            tws[aa] = twist32[lane * R1 + aa];
+            for (int aa = 0; aa < R1; ++aa) {
+                dw[aa] = pack_f16x2((float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]);
+                tws[aa] = fwd_t2[lane * R1 + aa];
+            }
+            fft512_fwd_mfma(v, dw, am, tws, tw, s_buf, lane);
+        } else {
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                v[aa] = C2<float>{(float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]};
+                tws[aa] = twist32[lane * R1 + aa];
+            }
+            fft512_dit<+1, true>(v, tw, (const C2<float>*)tws, s_buf, lane);
         }
-        fft512_dit<+1, true>(v, tw, (const C2<float>*)tws, s_buf, lane);
         double* o = out + g * N;
 #pragma unroll
         for (int s = 0; s < R1 / 8; ++s)
@@ -1467,6 +1507,45 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
         cis_pi_over_N((long long)64 * a, N, &re, &im);
         wa[2 * a] = re; wa[2 * a + 1] = im;
     }
+    // matrix-core pass 1 of the split forward transform (N = 1024, fft512_fwd_mfma): the real form of
+    // G[k1][a] = W_8^{a k1} w^{64a}, rows 2 k1 + (0 re, 1 im), columns 2 a + (0 re, 1 im), split
+    // into two FP16 pieces, and the pass-2 input twiddles W_64^{b k1} w^{8b + c} of lane c + 8 k1
+    std::vector<_Float16> fa(64 * 16);
+    std::vector<float> ft2(2 * 64 * 8);
+    if (N == 1024) {
+        double A[16][16];
+        for (int k1 = 0; k1 < 8; ++k1)
+            for (int a = 0; a < 8; ++a) {
+                double gr, gi;
+                cis_pi_over_N((long long)(2 * N / 8) * a * k1 + 64LL * a, N, &gr, &gi);
+                A[2 * k1][2 * a] = gr; A[2 * k1][2 * a + 1] = -gi;
+                A[2 * k1 + 1][2 * a] = gi; A[2 * k1 + 1][2 * a + 1] = gr;
+            }
+        // pieces P[0] = fp16(A), P[1] = fp16(A - P[0])
+        _Float16 P[2][16][16];
+        for (int m = 0; m < 16; ++m)
+            for (int k = 0; k < 16; ++k) {
+                P[0][m][k] = (_Float16)A[m][k];
+                P[1][m][k] = (_Float16)(A[m][k] - (double)P[0][m][k]);
+            }
+        // lane L: entries 0-7 the two pieces in the K = 16 layout (A[L & 15][4 (L >> 4) + i]),
+        // entries 8-15 the K = 32 layout with the pieces stacked along K (A[L & 15][8 (L >> 4) + i],
+        // columns 16-31 being piece 2)
+        for (int L = 0; L < 64; ++L) {
+            for (int p = 0; p < 2; ++p)
+                for (int i = 0; i < 4; ++i) fa[L * 16 + 4 * p + i] = P[p][L & 15][4 * (L >> 4) + i];
+            for (int i = 0; i < 8; ++i) {
+                const int k = 8 * (L >> 4) + i;
+                fa[L * 16 + 8 + i] = P[k >> 4][L & 15][k & 15];
+            }
+        }
+        for (int L = 0; L < 64; ++L)
+            for (int b = 0; b < 8; ++b) {
+                double re, im;
+                cis_pi_over_N((long long)(2 * N / 64) * (L >> 3) * b + 8LL * b + (L & 7), N, &re, &im);
+                ft2[2 * (L * 8 + b)] = (float)re; ft2[2 * (L * 8 + b) + 1] = (float)im;
+            }
+    }
     // two-wavefront form (N = 2048): pass-1 twiddles of the 512-point transform of wavefront v with
     // its lane twist w^{(1+4v) L} folded in, and its register twist constants w^{64 a (1+4v)}
     std::vector<double> t1w(2 * 2 * 64 * 8), rw(2 * 2 * 8), rws(2 * 2 * 8);
@@ -1519,6 +1598,8 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
     if ((e = up(&tw.untw64, untw.data(), untw.size() * 8)) != hipSuccess) return e;
     if ((e = up(&tw.wa64, wa.data(), wa.size() * 8)) != hipSuccess) return e;
     if (N == 1024 && (e = up(&tw.q4tw, q4.data(), q4.size() * 4)) != hipSuccess) return e;
+    if (N == 1024 && (e = up(&tw.fwd_a16, fa.data(), fa.size() * 2)) != hipSuccess) return e;
+    if (N == 1024 && (e = up(&tw.fwd_t2, ft2.data(), ft2.size() * 4)) != hipSuccess) return e;
     if (N == 2048) {
         if ((e = up(&tw.t1w64, t1w.data(), t1w.size() * 8)) != hipSuccess) return e;
         if ((e = up(&tw.t1w32, t1w32.data(), t1w32.size() * 4)) != hipSuccess) return e;
@@ -1531,7 +1612,7 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
 
 void free_twiddles(DevTwiddles& tw) {
     void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.t2dit32, tw.twist32, tw.untw64, tw.wa64,
-                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64, tw.q4tw};
+                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64, tw.q4tw, tw.fwd_a16, tw.fwd_t2};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -2924,6 +3005,8 @@ static hipError_t launch_boot_t(const DevKeyView& key, const DevTwiddles& tw, Pr
     A.wa64 = tw.wa64;
     A.twist32 = tw.twist32;
     A.untw64 = tw.untw64;
+    A.fwd_a16 = tw.fwd_a16;
+    A.fwd_t2 = tw.fwd_t2;
     if (prec == PREC_FP64) {
         A.bk = key.bk64;
         A.t1 = tw.t1_fold64;
@@ -3137,8 +3220,9 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
         if (N == 1024)
             hipLaunchKernelGGL((k_fft_test32<1024>), dim3(count), dim3(64), 0, st, out, in, inverse,
                                (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2dit32,
-                               (const C2<float>*)tw.twist32, (const C2<double>*)tw.untw64);
-        else if (N == 2048)
+                               (const C2<float>*)tw.twist32, (const C2<double>*)tw.untw64,
+                               (const f16x8_t*)tw.fwd_a16, (const C2<float>*)tw.fwd_t2);
+        else if (N == 2048 && inverse != 2)
             hipLaunchKernelGGL((k_fft_test2w<1>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w32, tw.t2_32,
                                (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64);
         else return hipErrorInvalidValue;

@@ -493,6 +493,134 @@ __device__ __forceinline__ void fft512_dit(C2<T>* v, const DitTw<T>& tw, const C
     dft8_tw<S, CJ, true>(v, tw.t3);
 }
 
+// ---- forward transform of gadget digits with its first pass on the matrix cores ----
+//
+// The forward transform's inputs are gadget digits: integers of at most 11 bits, exact in FP16.  Its
+// first pass (fft512_dit: the 8-point DFT over a of z_{64a+L} w^{64a+L}) is, per vector L, the
+// fixed complex 8x8 matrix G[k1][a] = W_8^{a k1} w^{64a} applied to the eight inputs, followed by
+// the vector's own factor w^L.  As a real 16x16 matrix A (rows 2 k1 + {re, im}, columns
+// 2 a + {re, im}) it multiplies 16 vectors at once on the matrix cores.  A is split into two FP16
+// pieces, A1 = fp16(A) and A2 = fp16(A - A1): the entries lie in [-1, 1], so |A - A1| <= 2^-12 is a
+// normal FP16 number and A1 + A2 is within 2^-24 of A, the FP32 level.  The digits are exact, the
+// products of FP16 numbers are exact in FP32, and the sums accumulate in FP32.  The factor
+// w^L = w^{8b + c} moves into the pass-2 input twiddles (fwd_t2[b] = W_64^{b k1} w^{8b+c} for lane
+// c + 8 k1; b = 0 is no longer 1).  A wavefront's 64 vectors are four groups of 16.
+//
+// Two instruction forms (FWD_MFMA_K32):
+//   1: one v_mfma_f32_16x16x32_f16 per group, the pieces stacked along K (columns 0-15 of the
+//      operand are A1, 16-31 are A2, and both halves of B hold the same 16 inputs);
+//   0: two v_mfma_f32_16x16x16_f16 per group, one per piece, accumulating into the same output.
+// Operand layouts (checked on gfx950), q = l >> 4, n0 = l & 15: K = 16: lane l holds
+// A[n0][4q + i] and B[4q + i][n0], i < 4; K = 32: A[n0][8q + i] and B[8q + i][n0], i < 8; both:
+// D[4q + i][n0], i < 4.  So lane (n0, q) of group g gets outputs k1 = 2q, 2q + 1 of vector 16g + n0.
+#ifndef FWD_MFMA_K32
+#define FWD_MFMA_K32 0
+#endif
+typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// This lane's entries of the pass-1 matrix pieces.  Table [64 lanes][16] FP16: entries 0-3 piece 1
+// and 4-7 piece 2 in the K = 16 layout, entries 8-15 the stacked K = 32 layout.
+struct FwdMfmaA {
+#if FWD_MFMA_K32
+    f16x8_t a;
+#else
+    f16x4_t a[2];
+#endif
+};
+
+__device__ __forceinline__ void load_fwd_mfma_a(FwdMfmaA& m, const f16x8_t* __restrict__ tab, int lane) {
+#if FWD_MFMA_K32
+    m.a = tab[lane * 2 + 1];
+#else
+    const f16x8_t t = tab[lane * 2];
+    m.a[0] = f16x4_t{t[0], t[1], t[2], t[3]};
+    m.a[1] = f16x4_t{t[4], t[5], t[6], t[7]};
+#endif
+}
+
+// two FP32 integers as an FP16 pair (exact below 2^11), x in the low half
+__device__ __forceinline__ uint32_t pack_f16x2(float x, float y) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x, y));
+}
+
+// Forward 512-point transform of digit inputs.  In: dw[a] = FP16 pair (Re, Im) of input a of this
+// lane's vector (the digits of coefficients L + 64a and L + 64a + 512), i.e. fft512_dit's inputs
+// before the twist; dw is overwritten.  Out: v in natural order, as fft512_dit<+1, true> with the
+// twist w^{64a+L}.  fwd_t2: the pass-2 table with the lane twist folded in; tw.t3 as for fft512_dit.
+// The operands move between lanes with v_permlane16_swap (rows 1 and 3 of its first operand
+// exchanged with rows 0 and 2 of its second; rows of 16 lanes) and v_permlane32_swap (the upper
+// half of its first operand with the lower half of its second).
+__device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, const FwdMfmaA& am,
+                                                const C2<float>* fwd_t2, const DitTw<float>& tw, C2<float>* buf,
+                                                int lane) {
+    const int q = lane >> 4, n0 = lane & 15;
+    f32x4_t d[4];
+#if FWD_MFMA_K32
+    // group g, lane (n0, q): the four inputs 4 (q & 1) + i of vector 16g + n0, from row g.  The swap
+    // of (dw[i], dw[4+i]) puts row r's dw[i] and dw[4+i] in rows (0, 1) of its first result (r even)
+    // or second (r odd), and of rows (2, 3) for r + 2; a self-swap of 32 then copies each half to
+    // both halves.
+    uint32_t bg[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const auto s16 = __builtin_amdgcn_permlane16_swap(dw[i], dw[4 + i], false, false);
+        const auto e = __builtin_amdgcn_permlane32_swap(s16[0], s16[0], false, false);
+        const auto o = __builtin_amdgcn_permlane32_swap(s16[1], s16[1], false, false);
+        bg[0][i] = e[0];
+        bg[2][i] = e[1];
+        bg[1][i] = o[0];
+        bg[3][i] = o[1];
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const f16x8_t b = __builtin_bit_cast(f16x8_t, (uint4){bg[g][0], bg[g][1], bg[g][2], bg[g][3]});
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(am.a, b, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+#else
+    // lane bits 4, 5 <-> register bits 1, 2 of dw: lane (n0, q) then holds inputs 2q, 2q + 1 of
+    // vector 16g + n0 in dw[2g], dw[2g + 1]
+#pragma unroll
+    for (int r : {0, 1, 4, 5}) {
+        const auto s = __builtin_amdgcn_permlane16_swap(dw[r], dw[r + 2], false, false);
+        dw[r] = s[0];
+        dw[r + 2] = s[1];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const auto s = __builtin_amdgcn_permlane32_swap(dw[r], dw[r + 4], false, false);
+        dw[r] = s[0];
+        dw[r + 4] = s[1];
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const f16x4_t b = __builtin_bit_cast(f16x4_t, (uint2){dw[2 * g], dw[2 * g + 1]});
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(am.a[1], b, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(am.a[0], b, d[g], 0, 0, 0);
+    }
+#endif
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        // element (k1, vector 8b + c) of exchange 1 sits at x1_addr(k1, b, c) = 72 k1 + 8b + c
+        buf[(2 * q) * X1_ROW + 16 * g + n0] = C2<float>{d[g][0], d[g][1]};
+        buf[(2 * q + 1) * X1_ROW + 16 * g + n0] = C2<float>{d[g][2], d[g][3]};
+    }
+    wave_sync();
+    const int lo = lane & 7, hi = lane >> 3;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = buf[x1_addr(hi, b, lo)];
+    wave_sync();
+    dft8_tw<+1, false, false>(v, fwd_t2);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(0, k2, hi, lo)] = v[k2];
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[x2_addr(0, hi, lo, c)];
+    wave_sync();
+    dft8_tw<+1, false, true>(v, tw.t3);
+}
+
 // ---- integer stages ----
 
 // coefficient j of (X^a - 1) * p, a in [0, 2N): e = (j - a) mod 2N; (X^a p)_j = p_e or -p_{e-N}

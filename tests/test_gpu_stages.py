@@ -46,6 +46,33 @@ def test_fft_forward_matches_closed_form(prec, rtol, rng):
         assert err <= rtol * max(1.0, np.abs(ref).max()) * 8, (i, err)
 
 
+def test_fft_forward_digits_matrix_core_matches_closed_form(rng):
+    """The split path's forward transform of gadget digits, first pass on the matrix cores (two FP16
+    pieces of the twisted 8-point DFT matrix, FP32 accumulation), against direct evaluation at the
+    roots of X^N + 1.  Inputs: random digits of both ring sizes' ranges, the extreme digits, unit
+    impulses at the ends of each half, and the largest inputs the entry point accepts; the bound is
+    the FP32 transform's (same as test_fft_forward_matches_closed_form at prec 1)."""
+    polys = np.concatenate([rng.integers(-64, 64, (4, N)), rng.integers(-128, 128, (2, N)),
+                            np.full((1, N), -64), np.full((1, N), 63), rng.choice([-2047, 2047], (1, N)),
+                            np.eye(N, dtype=np.int64)[[0, 1, 63, 64, M - 1, M, M + 1, N - 1]]]).astype(np.int32)
+    got = T().fft_forward_digits(polys)
+    ref_fp32 = T().fft_forward(polys, prec=1)
+    for i in range(polys.shape[0]):
+        ref = closed_form_transform(polys[i])
+        scale = max(1.0, np.abs(ref).max())
+        err = np.abs(got[i] - ref).max()
+        assert err <= 4e-7 * scale * 8, (i, err)
+        # and no worse than the all-vector-ALU FP32 transform by more than a small factor
+        assert err <= 4 * max(np.abs(ref_fp32[i] - ref).max(), 1e-7 * scale), (i, err)
+
+
+def test_fft_forward_digits_rejects_out_of_range_inputs():
+    polys = np.zeros((2, N), np.int32)
+    polys[1, 5] = 2048
+    with pytest.raises(T().TfheError, match="inputs must lie"):
+        T().fft_forward_digits(polys)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-3), (1, None)])
 def test_fft_roundtrip(prec, tol, rng):
     polys = rng.integers(-64, 64, (4, N)).astype(np.int32)
