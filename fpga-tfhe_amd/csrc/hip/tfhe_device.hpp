@@ -460,6 +460,16 @@ __device__ __forceinline__ void load_dit_tw(DitTw<T>& tw, const C2<T>* __restric
     }
 }
 
+// Exchange of lane bit 3 with a register bit for a pair of values (r0: register bit 0, r1: 1):
+// each lane sends its partner (lane ^ 8, DPP row_ror:8 within the row of 16) the value the partner
+// keeps and takes the partner's in its place.  s3: this lane's bit 3.
+__device__ __forceinline__ void swap_lane_bit3(uint32_t& r0, uint32_t& r1, bool s3) {
+    const uint32_t send = s3 ? r0 : r1;
+    const uint32_t got = (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xF, 0xF, true);  // row_ror:8
+    r0 = s3 ? got : r0;
+    r1 = s3 ? r1 : got;
+}
+
 // 512-point transform with natural order in and out (lane L, register r <-> index L + 64 r), for
 // both directions: S = +1 computes F_k = sum_j z_j W_512^{jk}; S = -1 runs the same factorisation
 // with every root conjugated, which gives M times the inverse.  Decimation in time over
@@ -516,6 +526,11 @@ __device__ __forceinline__ void fft512_dit(C2<T>* v, const DitTw<T>& tw, const C
 #ifndef FWD_MFMA_K32
 #define FWD_MFMA_K32 0
 #endif
+// 1: the matrix-core outputs reach pass 2's layout in registers (one lane-bit exchange), 0: through
+// the LDS exchange buffer
+#ifndef FWD_MFMA_X1_REGS
+#define FWD_MFMA_X1_REGS 1
+#endif
 typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -551,7 +566,9 @@ __device__ __forceinline__ uint32_t pack_f16x2(float x, float y) {
 // twist w^{64a+L}.  fwd_t2: the pass-2 table with the lane twist folded in; tw.t3 as for fft512_dit.
 // The operands move between lanes with v_permlane16_swap (rows 1 and 3 of its first operand
 // exchanged with rows 0 and 2 of its second; rows of 16 lanes) and v_permlane32_swap (the upper
-// half of its first operand with the lower half of its second).
+// half of its first operand with the lower half of its second).  The outputs reach pass 2's layout
+// by one lane-bit exchange in registers (FWD_MFMA_X1_REGS) instead of exchange 1 through buf, which
+// is then used only by exchange 2.
 __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, const FwdMfmaA& am,
                                                 const C2<float>* fwd_t2, const DitTw<float>& tw, C2<float>* buf,
                                                 int lane) {
@@ -600,6 +617,25 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
         d[g] = __builtin_amdgcn_mfma_f32_16x16x16f16(am.a[0], b, d[g], 0, 0, 0);
     }
 #endif
+    const int lo = lane & 7, hi = lane >> 3;
+#if FWD_MFMA_X1_REGS
+    // Lane (n0, q) holds (k1 = 2q + t, b = 2g + (n0 >> 3)) in d[g][2t], d[g][2t + 1]; pass 2 wants lane
+    // c + 8 k1 to hold every b of its k1.  Only lane bit 3 (b's low bit here, k1's low bit there)
+    // and the register bit t trade places: each lane sends its partner (lane ^ 8, DPP row_ror:8
+    // within the row of 16) the half it gives up and takes the partner's in its place.
+    const bool s3 = (lane & 8) != 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            // (__builtin_bit_cast of an element of an ext_vector_type array reads element 0; the
+            // element is taken by value here)
+            uint32_t r0 = __float_as_uint(d[g][i]), r1 = __float_as_uint(d[g][2 + i]);
+            swap_lane_bit3(r0, r1, s3);
+            (i ? v[2 * g].y : v[2 * g].x) = __uint_as_float(r0);
+            (i ? v[2 * g + 1].y : v[2 * g + 1].x) = __uint_as_float(r1);
+        }
+#else
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         // element (k1, vector 8b + c) of exchange 1 sits at x1_addr(k1, b, c) = 72 k1 + 8b + c
@@ -607,10 +643,10 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
         buf[(2 * q + 1) * X1_ROW + 16 * g + n0] = C2<float>{d[g][2], d[g][3]};
     }
     wave_sync();
-    const int lo = lane & 7, hi = lane >> 3;
 #pragma unroll
     for (int b = 0; b < 8; ++b) v[b] = buf[x1_addr(hi, b, lo)];
     wave_sync();
+#endif
     dft8_tw<+1, false, false>(v, fwd_t2);
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(0, k2, hi, lo)] = v[k2];
