@@ -44,6 +44,10 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #ifndef TFHE_FWD_MFMA
 #define TFHE_FWD_MFMA 1
 #endif
+// the same in the split latency form (k_boot32lat)
+#ifndef TFHE_LAT_FWD_MFMA
+#define TFHE_LAT_FWD_MFMA 1
+#endif
 
 struct BootKArgs {
     BootLaunch L;
@@ -931,9 +935,17 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
 
     DitTw<float> tw;
     load_dit_tw(tw, (const C2<float>*)A.t2, (const C2<float>*)A.t1, lane);
+    // forward twist (fft512_dit), or the matrix-core form's pass-2 table and matrix pieces
     C2<float> tws[R1];
+    FwdMfmaA am;
+#if TFHE_LAT_FWD_MFMA
+#pragma unroll
+    for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.fwd_t2)[lane * R1 + r];
+    load_fwd_mfma_a(am, (const f16x8_t*)A.fwd_a16, lane);
+#else
 #pragma unroll
     for (int r = 0; r < R1; ++r) tws[r] = ((const C2<float>*)A.twist32)[lane * R1 + r];
+#endif
     const C2<double>* untw = (const C2<double>*)A.untw64;
     const __amdgpu_buffer_rsrc_t key = key_rsrc(A.bk, (uint32_t)A.n * ROWSET_B);
     const int sb = A.split_bits;
@@ -957,10 +969,18 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
             uint32_t u[NC];
             lat_rotate<8, OFF>(u, s_acc, q, a, lane);
             C2<float> v[R1];
+#if TFHE_LAT_FWD_MFMA
+            uint32_t dw[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa)
+                dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
+            fft512_fwd_mfma(v, dw, am, tws, tw, s_x[w], lane);
+#else
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
                 v[aa] = C2<float>{(float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl)};
             fft512_dit<+1, true>(v, tw, tws, s_x[w], lane);
+#endif
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 s_part[w][0][r * 64 + lane] = cmul(v[r], C2<float>{k0[r].x, k0[r].y});
