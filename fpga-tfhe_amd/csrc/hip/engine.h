@@ -34,6 +34,11 @@ struct DevTwiddles {
     void* rw64 = nullptr;        // [2][8] C2<double>: w^{64 a (1 + 4v)}
     void* rw32 = nullptr;        // [2][8] C2<float>
     void* rws64 = nullptr;       // [2][8] C2<double>: w^{64 a (1 + 4v)} / 1024 (split path, own untwist)
+    // N = 2048 split path, matrix-core form of each wavefront's forward transform
+    // (fft512_fwd2w_mfma): this lane's entries of the two FP16 pieces of parity v's 16x32 matrix, and
+    // the pass-1 twiddles in pass 2's layout (shared with the inverse, fft512_inv2w_t1p)
+    void* w2_a16 = nullptr;      // [2][64][2][8] FP16: piece p of A_v[L & 15][8 (L >> 4) + i]
+    void* w2_t1p32 = nullptr;    // [2][64][8] C2<float>: w^{(8b + (L&7)) (1 + 4v + 8 (L>>3))}
     // N = 1024, 4 bootstraps per wavefront (form 6): pass-2 twiddles [16][2][16] C2<float>
     void* q4tw = nullptr;
 };

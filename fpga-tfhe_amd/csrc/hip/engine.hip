@@ -48,6 +48,12 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #ifndef TFHE_LAT_FWD_MFMA
 #define TFHE_LAT_FWD_MFMA 1
 #endif
+// Split path at N = 2048 (k_boot2w): 1 = each wavefront's forward transforms take the digit
+// combination, register twist and first pass on the matrix cores (fft512_fwd2w_mfma), with the
+// pass-1 twiddles in pass 2's layout shared by the inverse (fft512_inv2w_t1p); 0 = fft_fwd / fft_inv
+#ifndef TFHE_W2_FWD_MFMA
+#define TFHE_W2_FWD_MFMA 1
+#endif
 
 struct BootKArgs {
     BootLaunch L;
@@ -1589,6 +1595,44 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
             }
         }
     }
+    // N = 2048 split path, matrix-core form of the forward (fft512_fwd2w_mfma): per parity v the
+    // real form of X[k1] = sum_a W_8^{a k1} w^{64a(1+4v)} (P0_a + c_v P1_a), c_v = (-1)^v e^{i pi/4},
+    // rows 2 k1 + (0 re, 1 im), columns 4a + (P0 re, P0 im, P1 re, P1 im), in two FP16 pieces, lane L
+    // holding A_v[L & 15][8 (L >> 4) + i]; and the pass-1 twiddles w^{L(1+4v+8k1)} in pass 2's layout
+    std::vector<_Float16> w2a(2 * 64 * 2 * 8);
+    std::vector<float> w2t(2 * 2 * 64 * 8);
+    if (N == 2048) {
+        for (int v = 0; v < 2; ++v) {
+            double A[16][32];
+            for (int k1 = 0; k1 < 8; ++k1)
+                for (int a = 0; a < 8; ++a) {
+                    const long long e0 = (long long)(2 * N / 8) * a * k1 + 64LL * a * (1 + 4 * v);
+                    double g0r, g0i, g1r, g1i;
+                    cis_pi_over_N(e0, N, &g0r, &g0i);
+                    cis_pi_over_N(e0 + N / 4 + (long long)N * v, N, &g1r, &g1i);   // times (-1)^v e^{i pi/4}
+                    A[2 * k1][4 * a + 0] = g0r; A[2 * k1][4 * a + 1] = -g0i;
+                    A[2 * k1][4 * a + 2] = g1r; A[2 * k1][4 * a + 3] = -g1i;
+                    A[2 * k1 + 1][4 * a + 0] = g0i; A[2 * k1 + 1][4 * a + 1] = g0r;
+                    A[2 * k1 + 1][4 * a + 2] = g1i; A[2 * k1 + 1][4 * a + 3] = g1r;
+                }
+            for (int L = 0; L < 64; ++L)
+                for (int i = 0; i < 8; ++i) {
+                    const double x = A[L & 15][8 * (L >> 4) + i];
+                    const _Float16 h0 = (_Float16)x;
+                    const _Float16 h1 = (_Float16)(x - (double)h0);
+                    w2a[((v * 64 + L) * 2 + 0) * 8 + i] = h0;
+                    w2a[((v * 64 + L) * 2 + 1) * 8 + i] = h1;
+                }
+            for (int L = 0; L < 64; ++L)
+                for (int b = 0; b < 8; ++b) {
+                    double re, im;
+                    const long long Lv = 8 * b + (L & 7), k1 = L >> 3;
+                    cis_pi_over_N(Lv * (1 + 4 * v + 8 * k1), N, &re, &im);
+                    w2t[2 * ((v * 64 + L) * 8 + b)] = (float)re;
+                    w2t[2 * ((v * 64 + L) * 8 + b) + 1] = (float)im;
+                }
+        }
+    }
     // 4-bootstraps-per-wavefront form (N = 1024): pass-2 twiddles T[i][l + 16 s] =
     // e^{i pi i (4 (l + 16 s) + 1) / N}, table [l < 16][s < 2][i < 16]
     std::vector<float> q4(2 * 16 * 2 * 16);
@@ -1626,13 +1670,15 @@ hipError_t build_twiddles(DevTwiddles& tw, int N) {
         if ((e = up(&tw.rw64, rw.data(), rw.size() * 8)) != hipSuccess) return e;
         if ((e = up(&tw.rw32, rw32.data(), rw32.size() * 4)) != hipSuccess) return e;
         if ((e = up(&tw.rws64, rws.data(), rws.size() * 8)) != hipSuccess) return e;
+        if ((e = up(&tw.w2_a16, w2a.data(), w2a.size() * 2)) != hipSuccess) return e;
+        if ((e = up(&tw.w2_t1p32, w2t.data(), w2t.size() * 4)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 void free_twiddles(DevTwiddles& tw) {
     void* ps[] = {tw.t1_fold64, tw.t2_64, tw.t1_plain32, tw.t2_32, tw.t2dit32, tw.twist32, tw.untw64, tw.wa64,
-                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64, tw.q4tw, tw.fwd_a16, tw.fwd_t2};
+                  tw.t1w64, tw.t1w32, tw.rw64, tw.rw32, tw.rws64, tw.q4tw, tw.fwd_a16, tw.fwd_t2, tw.w2_a16, tw.w2_t1p32};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     tw = DevTwiddles();
@@ -1800,8 +1846,8 @@ constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4;
 template <int L, int BG, bool ROT>
 __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const void* sx_other, int a,
                                          const float4* __restrict__ kr, const LaneTw<float, 8>& tw, const C2<float>* __restrict__ rw32,
-                                         const C2<double>* __restrict__ rws, int wave, int lane, int split_bits,
-                                         double* raw, double* raw_hi, bool work) {
+                                         const C2<double>* __restrict__ rws, const f16x8_t* a2, int wave, int lane,
+                                         int split_bits, double* raw, double* raw_hi, bool work) {
     constexpr int N = W2_N, M = W2_M, R1 = 8, NC = 32;
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     C2<float>* sbuf = (C2<float>*)sx_own;
@@ -1817,6 +1863,18 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     if (work) {
     uint32_t u[NC];
     auto rotate = [&](int q) { w2_rotate<W2_G_SPLIT, ROT, OFF>(u, s_acc, q, a, lane); };
+#if TFHE_W2_FWD_MFMA
+    // the row's digits as FP16 pairs P0_a = (p_{j'}, p_{j'+1024}), P1_a = (p_{j'+512}, p_{j'+1536})
+    (void)hc;
+    (void)rw32;
+    auto prep = [&](uint32_t* x0, uint32_t* x1, int lvl) {
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) {
+            x0[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl));
+            x1[aa] = pack_f16x2((float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl));
+        }
+    };
+#else
     auto prep = [&](C2<float>* x, int lvl) {
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa)
@@ -1824,16 +1882,24 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                               (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl), hc,
                               rw32[aa], aa == 0);
     };
+#endif
     // rows t = (q, lvl) in order, software-pipelined: the row's first key polynomial is requested
     // before its transform, the second after it, and the next row's digits are prepared (rotating
     // the next polynomial first when q changes) while the second polynomial is in flight
     C2<float> v[R1];
+#if TFHE_W2_FWD_MFMA
+    uint32_t x0[R1], x1[R1];
+#endif
     float4 k0[R1];
     // row 0's first key polynomial is requested before the rotation, which then hides it too
 #pragma unroll
     for (int r = 0; r < R1; ++r) k0[r] = kr[wave * W2_H + lane + r * 64];
     rotate(0);
+#if TFHE_W2_FWD_MFMA
+    prep(x0, x1, 0);
+#else
     prep(v, 0);
+#endif
 #pragma unroll
     for (int row = 0; row < 2 * L; ++row) {
         const float4* k = kr + (size_t)(row * 2) * M + wave * W2_H + lane;
@@ -1841,7 +1907,11 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
 #pragma unroll
             for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
         }
+#if TFHE_W2_FWD_MFMA
+        fft512_fwd2w_mfma(v, x0, x1, a2, tw.t1, tw.t2, sbuf, lane);
+#else
         fft_fwd<R1>(v, tw, sbuf, lane);
+#endif
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
             cmac(h0[r], v[r], C2<float>{k0[r].x, k0[r].y});
@@ -1850,11 +1920,18 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
         float4 k1[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r) k1[r] = k[M + r * 64];
+#if TFHE_W2_FWD_MFMA
+        if (row + 1 < 2 * L) {
+            if ((row + 1) % L == 0) rotate((row + 1) / L);
+            prep(x0, x1, (row + 1) % L);
+        }
+#else
         C2<float> vn[R1];
         if (row + 1 < 2 * L) {
             if ((row + 1) % L == 0) rotate((row + 1) / L);
             prep(vn, (row + 1) % L);
         }
+#endif
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
             cmac(h1[r], v[r], C2<float>{k1[r].x, k1[r].y});
@@ -1864,15 +1941,24 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
         // cost 4.5% once the key requests moved ahead of the transforms; without the barrier the
         // step is 6% slower.  (The FP64 step keeps its warm-up loads: 30% slower without.)
         __builtin_amdgcn_sched_barrier(0);
+#if !TFHE_W2_FWD_MFMA
         if (row + 1 < 2 * L) {
 #pragma unroll
             for (int r = 0; r < R1; ++r) v[r] = vn[r];
         }
+#endif
     }
+#if TFHE_W2_FWD_MFMA
+    fft512_inv2w_t1p(h0, tw.t1, tw.t2, sbuf, lane);
+    fft512_inv2w_t1p(l0, tw.t1, tw.t2, sbuf, lane);
+    fft512_inv2w_t1p(h1, tw.t1, tw.t2, sbuf, lane);
+    fft512_inv2w_t1p(l1, tw.t1, tw.t2, sbuf, lane);
+#else
     fft_inv<R1>(h0, tw, sbuf, lane);
     fft_inv<R1>(l0, tw, sbuf, lane);
     fft_inv<R1>(h1, tw, sbuf, lane);
     fft_inv<R1>(l1, tw, sbuf, lane);
+#endif
     }
     const double hs = ldexp(1.0, split_bits);
     // one output polynomial at a time: exchange hi, combine and round it, then lo.  The
@@ -2067,7 +2153,15 @@ k_boot2w(BootKArgs A) {
     const size_t rs = (A.L.mode == 2 && active) ? (size_t)A.L.row_set[g] * ROWSET : 0;
     if constexpr (P == 1) {
         LaneTw<float, 8> tw;
+        f16x8_t a2[2];
+#if TFHE_W2_FWD_MFMA
+        // t1: the pass-1 twiddles in pass 2's layout, and the matrix-core pieces of this parity
+        load_lane_tw(tw, (const C2<float>*)A.fwd_t2 + wave * 64 * 8, (const C2<float>*)A.t2, lane);
+        a2[0] = ((const f16x8_t*)A.fwd_a16)[(wave * 64 + lane) * 2];
+        a2[1] = ((const f16x8_t*)A.fwd_a16)[(wave * 64 + lane) * 2 + 1];
+#else
         load_lane_tw(tw, (const C2<float>*)A.t1 + wave * 64 * 8, (const C2<float>*)A.t2, lane);
+#endif
         const C2<float>* rw32 = (const C2<float>*)A.t2w_rw32 + wave * 8;
         const C2<double>* rws = (const C2<double>*)A.untw64 + wave * 8;
         void* sx_own = s_xd[b][wave];
@@ -2076,14 +2170,14 @@ k_boot2w(BootKArgs A) {
         if (A.L.mode == 2) {
             double* raw = (A.L.raw_out && active) ? A.L.raw_out + (size_t)g * 2 * N : nullptr;
             double* raw_hi = (A.L.raw_hi && active) ? A.L.raw_hi + (size_t)g * 2 * N : nullptr;
-            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, tw, rw32, rws, wave, lane,
+            step2w32<L, BG, false>(s_acc, sx_own, sx_other, 0, key + rs, tw, rw32, rws, a2, wave, lane,
                                    A.split_bits, raw, raw_hi, active);
         } else {
             for (int i = 0; i < A.n; ++i) {
                 // a zero rotation skips the step (tfhe_blindRotate_FFT.cpp:114); a is the same in both
                 // wavefronts of a bootstrap, and an idle bootstrap still meets every barrier
                 const int a = active ? __builtin_amdgcn_readfirstlane((int)s_bara[i]) : 0;
-                step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET, tw, rw32, rws, wave, lane,
+                step2w32<L, BG, true>(s_acc, sx_own, sx_other, a, key + (size_t)i * ROWSET, tw, rw32, rws, a2, wave, lane,
                                       A.split_bits, nullptr, nullptr, a != 0);
             }
         }
@@ -2426,12 +2520,16 @@ __global__ void __launch_bounds__(128) k_keyprep2w(const int32_t* __restrict__ b
 // Transform test kernel at N = 2048 (two wavefronts): forward int32[N] -> F_k natural order, or
 // inverse F natural -> polynomial (FP64 values), with the bootstrap's prep and output pairing of
 // each path (P = 1: split, whose wavefronts untwist their own outputs with rws64 and exchange them).
+// inverse: 0 forward, 1 inverse; for P = 1 (split) 2 is the forward of digit inputs (|p_j| < 2048)
+// with the matrix-core form (fft512_fwd2w_mfma), and the inverse is the one the bootstrap kernel
+// runs (fft512_inv2w_t1p with the pass-2-layout table t1p when TFHE_W2_FWD_MFMA is 1).
 template <int P>
 __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, const void* __restrict__ in, int inverse,
                                                     const void* __restrict__ t1w, const void* __restrict__ t2,
                                                     const C2<float>* __restrict__ rw32,
                                                     const C2<double>* __restrict__ rw64,
-                                                    const C2<double>* __restrict__ rws64) {
+                                                    const C2<double>* __restrict__ rws64,
+                                                    const void* __restrict__ w2_a16, const void* __restrict__ t1p) {
     using T = typename std::conditional<P == 1, float, double>::type;
     constexpr int N = W2_N, R1 = 8;
     __shared__ C2<double> s_buf[2][xbuf_elems<W2_H>()];
@@ -2439,10 +2537,33 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const size_t g = blockIdx.x;
     LaneTw<T, R1> tw;
-    load_lane_tw(tw, (const C2<T>*)t1w + wave * 64 * 8, (const C2<T>*)t2, lane);
+    constexpr bool MF = P == 1 && TFHE_W2_FWD_MFMA;
+    const bool use_t1p = P == 1 && (inverse == 2 || (inverse == 1 && MF));
+    load_lane_tw(tw, (const C2<T>*)(use_t1p ? t1p : t1w) + wave * 64 * 8, (const C2<T>*)t2, lane);
     C2<T> v[R1];
     double* o = out + g * N;
-    if (!inverse) {
+    if (inverse == 2) {
+        if constexpr (P == 1) {
+            const int32_t* p = (const int32_t*)in + g * N;
+            uint32_t x0[R1], x1[R1];
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) {
+                const int j = lane + 64 * aa;
+                x0[aa] = pack_f16x2((float)p[j], (float)p[j + 1024]);
+                x1[aa] = pack_f16x2((float)p[j + 512], (float)p[j + 1536]);
+            }
+            f16x8_t a2[2];
+            a2[0] = ((const f16x8_t*)w2_a16)[(wave * 64 + lane) * 2];
+            a2[1] = ((const f16x8_t*)w2_a16)[(wave * 64 + lane) * 2 + 1];
+            fft512_fwd2w_mfma(v, x0, x1, a2, tw.t1, tw.t2, (C2<float>*)s_buf[wave], lane);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const int k = 2 * (lane + 64 * r) + wave;
+                o[2 * k] = v[r].x;
+                o[2 * k + 1] = v[r].y;
+            }
+        }
+    } else if (!inverse) {
         const int32_t* p = (const int32_t*)in + g * N;
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
@@ -2469,7 +2590,8 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
             const int k = 2 * (lane + 64 * r) + wave;
             v[r] = C2<T>{(T)F[2 * k], (T)F[2 * k + 1]};
         }
-        fft_inv<R1>(v, tw, s_buf[wave], lane);
+        if constexpr (MF) fft512_inv2w_t1p((C2<float>*)v, tw.t1, tw.t2, (C2<float>*)s_buf[wave], lane);
+        else fft_inv<R1>(v, tw, s_buf[wave], lane);
         __syncthreads();
         C2<double>* xo = (C2<double>*)s_buf[wave];
         const C2<double>* xp = (const C2<double>*)s_buf[1 - wave];
@@ -3101,6 +3223,8 @@ hipError_t launch_bootstrap(const DevKeyView& key, const DevTwiddles& tw, Prec p
         A.split_bits = key.split_bits;
         A.t2w_rw64 = tw.rw64;
         A.t2w_rw32 = tw.rw32;
+        A.fwd_a16 = tw.w2_a16;
+        A.fwd_t2 = tw.w2_t1p32;
         // launches of at most one bootstrap per CU take the latency form (FP64: unless a launch form
         // other than it is forced, as at N = 1024).  Forms: 3 latency; 2, 4, 5 the batched kernel
         // with 1, 2, 4 bootstraps per workgroup; 0 by launch size (2 per workgroup).  Two per
@@ -3234,7 +3358,7 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<double>*)tw.t1_fold64, (const C2<double>*)tw.t2_64, (const C2<double>*)tw.wa64);
         else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test2w<0>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w64, tw.t2_64,
-                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64);
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64, nullptr, nullptr);
         else return hipErrorInvalidValue;
     } else {
         if (N == 1024)
@@ -3242,9 +3366,9 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
                                (const C2<float>*)tw.t1_plain32, (const C2<float>*)tw.t2dit32,
                                (const C2<float>*)tw.twist32, (const C2<double>*)tw.untw64,
                                (const f16x8_t*)tw.fwd_a16, (const C2<float>*)tw.fwd_t2);
-        else if (N == 2048 && inverse != 2)
+        else if (N == 2048)
             hipLaunchKernelGGL((k_fft_test2w<1>), dim3(count), dim3(128), 0, st, out, in, inverse, tw.t1w32, tw.t2_32,
-                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64);
+                               (const C2<float>*)tw.rw32, (const C2<double>*)tw.rw64, (const C2<double>*)tw.rws64, tw.w2_a16, tw.w2_t1p32);
         else return hipErrorInvalidValue;
     }
     return hipGetLastError();

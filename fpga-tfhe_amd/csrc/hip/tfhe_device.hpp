@@ -560,6 +560,41 @@ __device__ __forceinline__ uint32_t pack_f16x2(float x, float y) {
     return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x, y));
 }
 
+// Lane bits 4 and 5 trade places with register bits 1 and 2 of dw[8] (v_permlane16_swap of
+// registers (r, r + 2), then v_permlane32_swap of (r, r + 4)).
+__device__ __forceinline__ void swap_lane45_reg12(uint32_t* dw) {
+#pragma unroll
+    for (int r : {0, 1, 4, 5}) {
+        const auto t = __builtin_amdgcn_permlane16_swap(dw[r], dw[r + 2], false, false);
+        dw[r] = t[0];
+        dw[r + 2] = t[1];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const auto t = __builtin_amdgcn_permlane32_swap(dw[r], dw[r + 4], false, false);
+        dw[r] = t[0];
+        dw[r + 4] = t[1];
+    }
+}
+
+// MFMA accumulators d[g] (lane (n0, q) holds (k1 = 2q + t) of vector 16g + n0 in d[g][2t],
+// d[g][2t + 1]) to pass 2's layout (lane c + 8 k1 holds every b of its k1, v[b]): one lane-bit
+// exchange (lane bit 3 with the register bit t, swap_lane_bit3).
+__device__ __forceinline__ void mfma_out_to_pass2(C2<float>* v, const f32x4_t* d, int lane) {
+    const bool s3 = (lane & 8) != 0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            // (__builtin_bit_cast of an element of an ext_vector_type array reads element 0; the
+            // element is taken by value here)
+            uint32_t r0 = __float_as_uint(d[g][i]), r1 = __float_as_uint(d[g][2 + i]);
+            swap_lane_bit3(r0, r1, s3);
+            (i ? v[2 * g].y : v[2 * g].x) = __uint_as_float(r0);
+            (i ? v[2 * g + 1].y : v[2 * g + 1].x) = __uint_as_float(r1);
+        }
+}
+
 // Forward 512-point transform of digit inputs.  In: dw[a] = FP16 pair (Re, Im) of input a of this
 // lane's vector (the digits of coefficients L + 64a and L + 64a + 512), i.e. fft512_dit's inputs
 // before the twist; dw is overwritten.  Out: v in natural order, as fft512_dit<+1, true> with the
@@ -572,7 +607,6 @@ __device__ __forceinline__ uint32_t pack_f16x2(float x, float y) {
 __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, const FwdMfmaA& am,
                                                 const C2<float>* fwd_t2, const DitTw<float>& tw, C2<float>* buf,
                                                 int lane) {
-    const int q = lane >> 4, n0 = lane & 15;
     f32x4_t d[4];
 #if FWD_MFMA_K32
     // group g, lane (n0, q): the four inputs 4 (q & 1) + i of vector 16g + n0, from row g.  The swap
@@ -598,18 +632,7 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
 #else
     // lane bits 4, 5 <-> register bits 1, 2 of dw: lane (n0, q) then holds inputs 2q, 2q + 1 of
     // vector 16g + n0 in dw[2g], dw[2g + 1]
-#pragma unroll
-    for (int r : {0, 1, 4, 5}) {
-        const auto s = __builtin_amdgcn_permlane16_swap(dw[r], dw[r + 2], false, false);
-        dw[r] = s[0];
-        dw[r + 2] = s[1];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const auto s = __builtin_amdgcn_permlane32_swap(dw[r], dw[r + 4], false, false);
-        dw[r] = s[0];
-        dw[r + 4] = s[1];
-    }
+    swap_lane45_reg12(dw);
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         const f16x4_t b = __builtin_bit_cast(f16x4_t, (uint2){dw[2 * g], dw[2 * g + 1]});
@@ -621,21 +644,10 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
 #if FWD_MFMA_X1_REGS
     // Lane (n0, q) holds (k1 = 2q + t, b = 2g + (n0 >> 3)) in d[g][2t], d[g][2t + 1]; pass 2 wants lane
     // c + 8 k1 to hold every b of its k1.  Only lane bit 3 (b's low bit here, k1's low bit there)
-    // and the register bit t trade places: each lane sends its partner (lane ^ 8, DPP row_ror:8
-    // within the row of 16) the half it gives up and takes the partner's in its place.
-    const bool s3 = (lane & 8) != 0;
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            // (__builtin_bit_cast of an element of an ext_vector_type array reads element 0; the
-            // element is taken by value here)
-            uint32_t r0 = __float_as_uint(d[g][i]), r1 = __float_as_uint(d[g][2 + i]);
-            swap_lane_bit3(r0, r1, s3);
-            (i ? v[2 * g].y : v[2 * g].x) = __uint_as_float(r0);
-            (i ? v[2 * g + 1].y : v[2 * g + 1].x) = __uint_as_float(r1);
-        }
+    // and the register bit t trade places.
+    mfma_out_to_pass2(v, d, lane);
 #else
+    const int q = lane >> 4, n0 = lane & 15;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         // element (k1, vector 8b + c) of exchange 1 sits at x1_addr(k1, b, c) = 72 k1 + 8b + c
@@ -655,6 +667,79 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
     for (int c = 0; c < 8; ++c) v[c] = buf[x2_addr(0, hi, lo, c)];
     wave_sync();
     dft8_tw<+1, false, true>(v, tw.t3);
+}
+
+// N = 2048, split path: the 512-point forward transform of wavefront v's half of the spectrum,
+// y_v[j'] = w^{(1+4v) j'} (P_{j'} + c_v P_{j'+512}), P_j = p_j + i p_{j+1024}, c_v = (-1)^v e^{i pi/4}
+// (the decimation-in-frequency fft_fwd<8> with its pass-1 twiddles w^{L(1+4v+8k1)}).  The
+// combination P0 + c_v P1, the register twist w^{64a(1+4v)} and the first 8-point pass form, per
+// vector L, a fixed complex 8x16 matrix on the 16 digit pairs (P0_a, P1_a); as a real 16x32 matrix
+// (rows 2 k1 + {re, im}, columns 4a + {P0 re, P0 im, P1 re, P1 im}) split into two FP16 pieces it
+// runs as two v_mfma_f32_16x16x32_f16 per group of 16 vectors.  Operand layouts (checked on
+// gfx950), q = l >> 4, n0 = l & 15: lane l holds A[n0][8q + i] and B[8q + i][n0], i < 8, and
+// D[4q + i][n0], i < 4.  In: p0[a], p1[a] = FP16 pairs of P0_a, P1_a of this lane's vector
+// (overwritten).  a2: this lane's entries of the two pieces.  t1p: the pass-1 twiddles in pass 2's
+// layout (lane c + 8 k1, register b: w^{L(1+4v+8k1)}, L = 8b + c), applied on pass 2's inputs.
+// Out: v in lane order, as fft_fwd<8> of y_v.
+__device__ __forceinline__ void fft512_fwd2w_mfma(C2<float>* v, uint32_t* p0, uint32_t* p1, const f16x8_t* a2,
+                                                  const C2<float>* t1p, const C2<float>* t2, C2<float>* buf,
+                                                  int lane) {
+    // lane (n0, q) of group g needs inputs 2q, 2q + 1 of vector 16g + n0
+    swap_lane45_reg12(p0);
+    swap_lane45_reg12(p1);
+    f32x4_t d[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const f16x8_t b = __builtin_bit_cast(f16x8_t, (uint4){p0[2 * g], p1[2 * g], p0[2 * g + 1], p1[2 * g + 1]});
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2[1], b, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        d[g] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2[0], b, d[g], 0, 0, 0);
+    }
+    mfma_out_to_pass2(v, d, lane);
+    const int c = lane & 7, k1lo = lane >> 3;
+    dft8_tw<+1, false, false>(v, t1p);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) v[k2] = cmul(v[k2], t2[k2]);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) buf[x2_addr(0, k2, k1lo, c)] = v[k2];
+    wave_sync();
+    {
+        const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) v[cc] = buf[x2_addr(0, k2, k1lo3, cc)];
+    }
+    wave_sync();
+    dft8<+1, float>(v);
+}
+
+// The inverse of fft512_fwd2w_mfma's transform (fft_inv<8> of the split N = 2048 path) with the
+// conjugate pass-1 twiddles taken from the same pass-2-layout table t1p, applied before exchange 1
+// instead of on the last pass's inputs.
+__device__ __forceinline__ void fft512_inv2w_t1p(C2<float>* v, const C2<float>* t1p, const C2<float>* t2,
+                                                 C2<float>* buf, int lane) {
+    dft8<-1, float>(v);
+    const int c = lane & 7, k1lo = lane >> 3;
+    {
+        const int k1lo3 = lane & 7, k2 = lane >> 3;
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc) buf[x2_addr(0, k2, k1lo3, cc)] = v[cc];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) v[k2] = buf[x2_addr(0, k2, k1lo, c)];
+    wave_sync();
+    dft8_tw<-1, true, true>(v, t2);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = cmulc(v[b], t1p[b]);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) buf[x1_addr(k1lo, b, c)] = v[b];
+    wave_sync();
+    {
+        const int b1 = lane >> 3, c1 = lane & 7;
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) v[k1] = buf[x1_addr(k1, b1, c1)];
+    }
+    wave_sync();
+    dft8<-1, float>(v);
 }
 
 // ---- integer stages ----

@@ -943,7 +943,7 @@ static int fft_stage(int32_t N, int32_t prec, int inverse, double* out, const vo
     if ((rc = ensure_twiddles(N, &tw))) return rc;
     if (inverse == 2) {
         // the matrix-core form takes gadget-digit inputs, exact in FP16
-        if (N != 1024 || prec != TFHE_PREC_SPLIT32) return arg_fail("digit transform: N = 1024, split FP32 only");
+        if (prec != TFHE_PREC_SPLIT32) return arg_fail("digit transform: split FP32 only");
         const int32_t* p = (const int32_t*)in;
         for (size_t i = 0; i < (size_t)count * N; ++i)
             if (p[i] <= -2048 || p[i] >= 2048) return arg_fail("digit transform: inputs must lie in (-2048, 2048)");
@@ -980,8 +980,8 @@ extern "C" int tfhe_gpu_fft_forward_batch(int32_t N, int32_t prec, double* out, 
 extern "C" int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, const double* in, int32_t count) {
     return fft_stage(N, prec, 1, out, in, count);
 }
-extern "C" int tfhe_gpu_fft_forward_digits_batch(double* out, const int32_t* in, int32_t count) {
-    return fft_stage(1024, TFHE_PREC_SPLIT32, 2, out, in, count);
+extern "C" int tfhe_gpu_fft_forward_digits_batch(int32_t N, double* out, const int32_t* in, int32_t count) {
+    return fft_stage(N, TFHE_PREC_SPLIT32, 2, out, in, count);
 }
 
 // ---- device memory helpers ----

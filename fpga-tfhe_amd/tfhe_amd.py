@@ -122,7 +122,7 @@ def lib():
         "tfhe_gpu_key_split_bits": (ctypes.c_int, [_vp, P(i32)]),
         "tfhe_gpu_rotate_decompose_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, _i32p, _i32p, i32]),
         "tfhe_gpu_fft_forward_batch": (ctypes.c_int, [i32, i32, _f64p, _i32p, i32]),
-        "tfhe_gpu_fft_forward_digits_batch": (ctypes.c_int, [_f64p, _i32p, i32]),
+        "tfhe_gpu_fft_forward_digits_batch": (ctypes.c_int, [i32, _f64p, _i32p, i32]),
         "tfhe_gpu_fft_inverse_batch": (ctypes.c_int, [i32, i32, _f64p, _f64p, i32]),
         "tfhe_gpu_fft_q4_batch": (ctypes.c_int, [i32, _f64p, _vp, i32]),
         "tfhe_gpu_malloc": (ctypes.c_int, [P(_vp), ctypes.c_size_t]),
@@ -422,12 +422,12 @@ def fft_forward(polys, prec=PREC_FP64):
 
 
 def fft_forward_digits(polys):
-    """The split path's forward transform of gadget digits (N = 1024, |p_j| < 2048), its first pass
-    on the matrix cores, as the bootstrap kernel runs it."""
+    """The split path's forward transform of gadget digits (N = 1024 or 2048, |p_j| < 2048), its
+    first pass on the matrix cores, as the bootstrap kernels run it."""
     polys = np.ascontiguousarray(polys, np.int32)
     cnt, N = polys.shape
     out = np.zeros((cnt, N), np.float64)
-    _ck(lib().tfhe_gpu_fft_forward_digits_batch(f64p(out), i32p(polys), cnt), "fft_forward_digits")
+    _ck(lib().tfhe_gpu_fft_forward_digits_batch(N, f64p(out), i32p(polys), cnt), "fft_forward_digits")
     return out[:, 0::2] + 1j * out[:, 1::2]
 
 

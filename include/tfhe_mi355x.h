@@ -446,10 +446,11 @@ int tfhe_gpu_rotate_decompose_batch(const tfhe_flat_params* p, int32_t* rot, int
  * integer-unit doubles [count][N]. */
 int tfhe_gpu_fft_forward_batch(int32_t N, int32_t prec, double* out, const int32_t* in, int32_t count);
 int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, const double* in, int32_t count);
-/* The split-FP32 forward transform as the N = 1024 bootstrap kernel runs it on gadget digits, its
- * first pass on the matrix cores: int32 [count][1024] with every value in (-2048, 2048) -> F as
- * tfhe_gpu_fft_forward_batch.  No counterpart in the reference (a check of the device transform). */
-int tfhe_gpu_fft_forward_digits_batch(double* out, const int32_t* in, int32_t count);
+/* The split-FP32 forward transform as the bootstrap kernels run it on gadget digits, its first pass
+ * on the matrix cores (N = 1024 or 2048): int32 [count][N] with every value in (-2048, 2048) -> F
+ * as tfhe_gpu_fft_forward_batch.  No counterpart in the reference (a check of the device
+ * transform). */
+int tfhe_gpu_fft_forward_digits_batch(int32_t N, double* out, const int32_t* in, int32_t count);
 /* The same two transforms (N = 1024, FP32) in the layout of launch form 6, four polynomials per
  * wavefront with 16 lanes each: forward int32 [count][1024] -> F natural order (interleaved
  * doubles [count][1024]) when inverse = 0, else F -> integer-unit doubles [count][1024]. */

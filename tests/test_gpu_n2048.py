@@ -50,6 +50,57 @@ def test_fft_forward_n2048_matches_closed_form(prec, rtol):
         assert err <= rtol * max(1.0, np.abs(ref).max()) * 8, (i, err)
 
 
+def closed_form_transform(p):
+    """F_k = p(w^(4k+1)), w = e^{i pi/N}: direct evaluation at the roots of X^N + 1."""
+    j = np.arange(N)
+    k = np.arange(N // 2)
+    e = (j[None, :] * (4 * k + 1)[:, None]) % (2 * N)
+    return (np.asarray(p, np.float64)[None, :] * np.exp(1j * np.pi * e / N)).sum(axis=1)
+
+
+def test_fft_forward_digits_n2048_matrix_core_matches_closed_form():
+    """The split path's forward transform of gadget digits as the two-wavefront bootstrap kernel
+    runs it (first pass on the matrix cores: two FP16 pieces of the twisted 8-point DFT over the two
+    folded inputs, FP32 accumulation) against direct evaluation at the roots of X^N + 1: random
+    digits of the Bgbit = 8 range, the extreme digits, impulses at the ends of each quarter (each
+    lands on another matrix-core operand column), and the largest accepted inputs."""
+    rng = np.random.default_rng(2050)
+    polys = np.concatenate([rng.integers(-128, 128, (4, N)), np.full((1, N), -128), np.full((1, N), 127),
+                            rng.choice([-2047, 2047], (1, N)),
+                            np.eye(N, dtype=np.int64)[[0, 1, 63, 511, 512, 1023, 1024, 1535, 1536, N - 1]]
+                            ]).astype(np.int32)
+    got = T().fft_forward_digits(polys)
+    ref_fp32 = T().fft_forward(polys, prec=1)
+    for i in range(polys.shape[0]):
+        ref = closed_form_transform(polys[i])
+        scale = max(1.0, np.abs(ref).max())
+        err = np.abs(got[i] - ref).max()
+        assert err <= 4e-7 * scale * 8, (i, err)
+        assert err <= 4 * max(np.abs(ref_fp32[i] - ref).max(), 1e-7 * scale), (i, err)
+
+
+def test_fft_forward_digits_n2048_rejects_out_of_range_inputs():
+    polys = np.zeros((1, N), np.int32)
+    polys[0, N - 1] = -2048
+    with pytest.raises(T().TfheError, match="inputs must lie"):
+        T().fft_forward_digits(polys)
+
+
+@pytest.mark.parametrize("prec,rtol", [(0, 1e-12), (1, 4e-7)])
+def test_fft_inverse_n2048_matches_closed_form(prec, rtol):
+    """The inverse transform the bootstrap kernels run at N = 2048 (for the split path: pass 1's
+    twiddles in the matrix-core forward's layout, untwist in FP64) applied to exact spectra
+    (direct evaluation of known polynomials) returns the polynomials."""
+    rng = np.random.default_rng(2051)
+    polys = np.concatenate([rand_torus(rng, (2, N)), rng.integers(-128, 128, (2, N)).astype(np.int32),
+                            np.eye(N, dtype=np.int32)[[0, 1, 1023, 1024, N - 1]] * 1000])
+    F = np.stack([closed_form_transform(p) for p in polys])
+    back = T().fft_inverse(F, prec=prec)
+    for i in range(polys.shape[0]):
+        err = np.abs(back[i] - polys[i]).max()
+        assert err <= rtol * max(1.0, np.abs(polys[i]).max()) * 8, (i, err)
+
+
 def test_rotate_decompose_n2048_bit_exact():
     rng = np.random.default_rng(2049)
     acc = rand_torus(rng, (8, 2, N))
