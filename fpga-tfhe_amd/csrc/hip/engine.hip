@@ -54,6 +54,11 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #ifndef TFHE_W2_FWD_MFMA
 #define TFHE_W2_FWD_MFMA 1
 #endif
+// N = 2048 split kernel with the matrix-core forward: the rotation reads each lane's coefficients
+// in the matrix-core operand layout (w2_rotate_b), so no lane exchange precedes the matrix products.
+#ifndef TFHE_W2_BLAYOUT
+#define TFHE_W2_BLAYOUT TFHE_W2_FWD_MFMA
+#endif
 
 struct BootKArgs {
     BootLaunch L;
@@ -1836,8 +1841,50 @@ __device__ __forceinline__ void w2_rotate(uint32_t* u, const int32_t* s_acc, int
     }
 }
 
-// mirror group sizes of the two N = 2048 kernels (see w2_mirror)
-constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4;
+// mirror group sizes of the N = 2048 kernels (see w2_mirror): the split latency form, the FP64
+// kernel, and the split two-wavefront kernel (w2_rotate_b's 128-word mirrors, G = 3, or w2_rotate's)
+constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4, W2_G_SPLIT2W = TFHE_W2_BLAYOUT ? 3 : W2_G_SPLIT;
+
+// Register r < 32 of lane (n0, q) = (lane & 15, lane >> 4) in the matrix-core operand layout:
+// coefficient w2b_lane(lane) + w2b_off(r), i.e. point j' = 16 g + n0 + 64 (2 q + t) of quarter
+// r >> 3 (j' + 512 (r >> 3)), rr = r & 7 = 2 g + t.  So prep's pairs (u[rr], u[rr + 16]) and
+// (u[rr + 8], u[rr + 24]) are the FP16 operands of fft512_fwd2w_mfma<true> as they stand.
+__host__ __device__ constexpr int w2b_off(int r) { return 16 * ((r & 7) >> 1) + 64 * (r & 1) + 512 * (r >> 3); }
+__device__ __forceinline__ int w2b_lane(int lane) { return (lane & 15) + 128 * (lane >> 4); }
+
+// w2_rotate for the operand layout: u[r] = ((X^a - 1) p_q + offset) ^ offset at coefficient
+// j = w2b_lane(lane) + w2b_off(r).  With d = j0 - a (j0 = w2b_lane(lane)), register r reads p at
+// e = (d + w2b_off(r)) mod 2N, negated when bit 11 of d + w2b_off(r) is set.  Quarter k = r >> 3
+// reads from one base b_k = (d + 512 k) mod N at offsets w2b_off(r & 7) <= 112, so 128-word mirrors
+// (w2_mirror<3>) cover every read, p0's through the shifted base as in w2_rotate.  The reads of
+// one register put the four rows of 16 lanes 128 words apart.
+template <int MW, bool ROT, uint32_t OFF>
+__device__ __forceinline__ void w2_rotate_b(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
+    constexpr int N = W2_N, NC = 32;
+    static_assert(MW >= 112, "mirrors cover the largest in-quarter offset");
+    const int32_t* pq = s_acc + q * N;
+    const int j0 = w2b_lane(lane);
+    if constexpr (ROT) {
+        const int d = j0 - a;
+        const uint32_t x = (uint32_t)d << 20;   // bit 11 of d + offset -> bit 31
+        const int sh = q ? 0 : MW;
+        int base[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) base[k] = (d + 512 * k + sh) & (N - 1);
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            const uint32_t pj = (uint32_t)pq[j0 + w2b_off(r)];
+            const uint32_t sv = (uint32_t)pq[base[r >> 3] + w2b_off(r & 7) - sh];
+            const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)w2b_off(r) << 20)) >> 31);
+            uint32_t c = OFF - pj;
+            asm("" : "+v"(c));
+            u[r] = (((sv + m) ^ m) + c) ^ OFF;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[j0 + w2b_off(r)] + OFF) ^ OFF;
+    }
+}
 
 // Split-FP32 step at N = 2048, one wavefront's share.  ROT as in step32.  Each wavefront untwists
 // its own inverse outputs in FP64 (rws: register constants with the 1/1024) and the untwisted values
@@ -1862,7 +1909,11 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     // part in every workgroup barrier below
     if (work) {
     uint32_t u[NC];
-    auto rotate = [&](int q) { w2_rotate<W2_G_SPLIT, ROT, OFF>(u, s_acc, q, a, lane); };
+#if TFHE_W2_BLAYOUT
+    auto rotate = [&](int q) { w2_rotate_b<w2_mirror<W2_G_SPLIT2W>(), ROT, OFF>(u, s_acc, q, a, lane); };
+#else
+    auto rotate = [&](int q) { w2_rotate<W2_G_SPLIT2W, ROT, OFF>(u, s_acc, q, a, lane); };
+#endif
 #if TFHE_W2_FWD_MFMA
     // the row's digits as FP16 pairs P0_a = (p_{j'}, p_{j'+1024}), P1_a = (p_{j'+512}, p_{j'+1536})
     (void)hc;
@@ -1908,7 +1959,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
             for (int r = 0; r < R1; ++r) k0[r] = k[r * 64];
         }
 #if TFHE_W2_FWD_MFMA
-        fft512_fwd2w_mfma(v, x0, x1, a2, tw.t1, tw.t2, sbuf, lane);
+        fft512_fwd2w_mfma<TFHE_W2_BLAYOUT>(v, x0, x1, a2, tw.t1, tw.t2, sbuf, lane);
 #else
         fft_fwd<R1>(v, tw, sbuf, lane);
 #endif
@@ -2006,7 +2057,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
                         const int32_t n1 = (int32_t)((uint32_t)s_acc[j1] + c1);
                         s_acc[j0] = n0;
                         s_acc[j1] = n1;
-                        w2_mirror_update<W2_G_SPLIT>(s_acc, q, wave, aa, j0, j1, n0, n1);
+                        w2_mirror_update<W2_G_SPLIT2W>(s_acc, q, wave, aa, j0, j1, n0, n1);
                     } else {
                         s_acc[j0] = (int32_t)c0;
                         s_acc[j1] = (int32_t)c1;
@@ -2120,7 +2171,7 @@ k_boot2w(BootKArgs A) {
     constexpr int XB = xbuf_elems<W2_H>();
     constexpr int T2S = 9;   // row stride of the LDS pass-2 twiddles: the 8 rows start on distinct banks
     // the accumulators carry mirrors of their ends (w2_rotate)
-    constexpr int AM = w2_mirror<P ? W2_G_SPLIT : W2_G_FP64>();
+    constexpr int AM = w2_mirror<P ? W2_G_SPLIT2W : W2_G_FP64>();
     __shared__ int32_t s_acc_all[GB][2 * N + 2 * AM];
     // per wavefront C2<double>[XB]: FP64 transform exchanges; split-FP32 transform exchanges
     // (C2<float>[XB]) and its untwisted outputs, rounded to FP32, in two alternating halves
@@ -2139,7 +2190,7 @@ k_boot2w(BootKArgs A) {
         // no fence: each lane copies only coefficients it wrote itself (w2_mirror_fill), and a
         // fence here would count as a possible write to global memory, turning the step's uniform
         // constant loads into vector loads
-        w2_mirror_fill<P ? W2_G_SPLIT : W2_G_FP64>(s_acc, lane);
+        w2_mirror_fill<P ? W2_G_SPLIT2W : W2_G_FP64>(s_acc, lane);
     }
     if constexpr (P == 0) {
         // lane c of the pass-2 table holds W_64^{c k2}, k2 < 8
@@ -2548,14 +2599,15 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
             uint32_t x0[R1], x1[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
-                const int j = lane + 64 * aa;
+                // the coefficients of register aa as the bootstrap kernel reads them
+                const int j = TFHE_W2_BLAYOUT ? w2b_lane(lane) + w2b_off(aa) : lane + 64 * aa;
                 x0[aa] = pack_f16x2((float)p[j], (float)p[j + 1024]);
                 x1[aa] = pack_f16x2((float)p[j + 512], (float)p[j + 1536]);
             }
             f16x8_t a2[2];
             a2[0] = ((const f16x8_t*)w2_a16)[(wave * 64 + lane) * 2];
             a2[1] = ((const f16x8_t*)w2_a16)[(wave * 64 + lane) * 2 + 1];
-            fft512_fwd2w_mfma(v, x0, x1, a2, tw.t1, tw.t2, (C2<float>*)s_buf[wave], lane);
+            fft512_fwd2w_mfma<TFHE_W2_BLAYOUT>(v, x0, x1, a2, tw.t1, tw.t2, (C2<float>*)s_buf[wave], lane);
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 const int k = 2 * (lane + 64 * r) + wave;

@@ -680,13 +680,18 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
 // D[4q + i][n0], i < 4.  In: p0[a], p1[a] = FP16 pairs of P0_a, P1_a of this lane's vector
 // (overwritten).  a2: this lane's entries of the two pieces.  t1p: the pass-1 twiddles in pass 2's
 // layout (lane c + 8 k1, register b: w^{L(1+4v+8k1)}, L = 8b + c), applied on pass 2's inputs.
-// Out: v in lane order, as fft_fwd<8> of y_v.
+// Out: v in lane order, as fft_fwd<8> of y_v.  IN_B: the inputs are already in the operand layout
+// (lane (n0, q), register 2g + t: input 2q + t of vector 16g + n0), as the bootstrap kernel reads
+// them from the accumulator; otherwise lane L holds input a of vector L in register a.
+template <bool IN_B = false>
 __device__ __forceinline__ void fft512_fwd2w_mfma(C2<float>* v, uint32_t* p0, uint32_t* p1, const f16x8_t* a2,
                                                   const C2<float>* t1p, const C2<float>* t2, C2<float>* buf,
                                                   int lane) {
     // lane (n0, q) of group g needs inputs 2q, 2q + 1 of vector 16g + n0
-    swap_lane45_reg12(p0);
-    swap_lane45_reg12(p1);
+    if constexpr (!IN_B) {
+        swap_lane45_reg12(p0);
+        swap_lane45_reg12(p1);
+    }
     f32x4_t d[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
