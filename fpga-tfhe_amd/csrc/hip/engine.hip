@@ -333,6 +333,17 @@ __attribute__((amdgpu_waves_per_eu(TFHE_BOOT64_WAVES, TFHE_BOOT64_WAVES))) k_boo
 }
 
 // ---------------------------------------------------------------------------------------------
+// The matrix-core operand layout of the split kernels' digits.  Register r of lane (n0, q) =
+// (lane & 15, lane >> 4) holds coefficient bl_lane(lane) + bl_off(r): point j' = 16 g + n0 +
+// 64 (2 q + t), rr = r & 7 = 2 g + t, of the 512-coefficient block r >> 3 (j' + 512 (r >> 3)).  The
+// FP16 digit pairs of registers (rr, rr + 8) (N = 1024) or (rr, rr + 16) and (rr + 8, rr + 24)
+// (N = 2048) are then the B operand of the first-pass matrix products as they stand, and no lane
+// exchange precedes them.
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int bl_off(int r) { return 16 * ((r & 7) >> 1) + 64 * (r & 1) + 512 * (r >> 3); }
+__device__ __forceinline__ int bl_lane(int lane) { return (lane & 15) + 128 * (lane >> 4); }
+
+// ---------------------------------------------------------------------------------------------
 // Rotation at N = 1024 from an accumulator with mirrored ends, as w2_rotate does at N = 2048: the
 // MW = 64 (G - 1) words below p0 repeat p0's last MW coefficients and the MW past p1 repeat p1's
 // first.  Registers r = G g + rr (r < 16) read p[(b_g + 64 rr) mod N], b_g = (d + 64 G g) mod N,
@@ -368,6 +379,32 @@ __device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, in
         const uint32_t pj = (uint32_t)pq[lane + 64 * r];
         const uint32_t sv = (uint32_t)pq[base[r / G] + 64 * (r % G) - sh];
         const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
+        uint32_t c = OFF - pj;
+        asm("" : "+v"(c));
+        u[r] = (((sv + m) ^ m) + c) ^ OFF;
+    }
+}
+// lat_rotate for the operand layout (bl_off, r < 16): register r reads p at e = (d + bl_off(r))
+// mod 2N, d = bl_lane(lane) - a, negated when bit 10 of d + bl_off(r) is set.  Each half r >> 3
+// reads from one base at offsets bl_off(r & 7) <= 112, so 128-word mirrors (lat_mirror<3>) cover
+// every read.  The reads of one register put the four rows of 16 lanes 128 words apart.
+template <int MW, uint32_t OFF>
+__device__ __forceinline__ void lat_rotate_b(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
+    constexpr int N = 1024, NC = 16;
+    static_assert(MW >= 112, "mirrors cover the largest in-half offset");
+    const int32_t* pq = s_acc + q * N;
+    const int j0 = bl_lane(lane);
+    const int d = j0 - a;
+    const uint32_t x = (uint32_t)d << 21;   // bit 10 of d + offset -> bit 31
+    const int sh = q ? 0 : MW;
+    int base[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) base[k] = (d + 512 * k + sh) & (N - 1);
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        const uint32_t pj = (uint32_t)pq[j0 + bl_off(r)];
+        const uint32_t sv = (uint32_t)pq[base[r >> 3] + bl_off(r & 7) - sh];
+        const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)bl_off(r) << 21)) >> 31);
         uint32_t c = OFF - pj;
         asm("" : "+v"(c));
         u[r] = (((sv + m) ^ m) + c) ^ OFF;
@@ -490,8 +527,13 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot64lat(BootKArgs A) {
 // ---------------------------------------------------------------------------------------------
 constexpr int B64W_GATES = 8;
 constexpr int B64W_MAXN = 640;   // largest LWE dimension this form is launched for
+// the split kernel reads its digits in the matrix-core operand layout (lat_rotate_b) when its
+// forward transforms take the matrix-core first pass
+#ifndef K32_BLAYOUT
+#define K32_BLAYOUT (TFHE_FWD_MFMA && !FWD_MFMA_K32)
+#endif
 #ifndef K32_MIRROR_G
-#define K32_MIRROR_G 8   // mirror group size of the split kernel's accumulator (lat_rotate)
+#define K32_MIRROR_G (K32_BLAYOUT ? 3 : 8)   // mirror group size of the split kernel's accumulator
 #endif
 #ifndef B64W_MIRROR_G
 #define B64W_MIRROR_G 4   // mirror group size of the accumulators (lat_rotate)
@@ -697,12 +739,21 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         uint32_t u[NC];
+#if K32_BLAYOUT
+        if constexpr (ROT) {
+            lat_rotate_b<lat_mirror<K32_MIRROR_G>(), OFF>(u, s_acc, q, a, lane);
+        } else {
+#pragma unroll
+            for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)s_acc[q * N + bl_lane(lane) + bl_off(r)] + OFF) ^ OFF;
+        }
+#else
         if constexpr (ROT) {
             lat_rotate<K32_MIRROR_G, OFF>(u, s_acc, q, a, lane);
         } else {
 #pragma unroll
             for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)s_acc[q * N + lane + 64 * r] + OFF) ^ OFF;
         }
+#endif
 #pragma unroll
         for (int lvl = 0; lvl < L; ++lvl) {
             C2<float> v[R1];
@@ -712,7 +763,7 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
                 dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
-            fft512_fwd_mfma(v, dw, am, tws, tw, s_buf, lane);
+            fft512_fwd_mfma<K32_BLAYOUT>(v, dw, am, tws, tw, s_buf, lane);
 #else
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
@@ -1425,10 +1476,12 @@ __global__ void __launch_bounds__(64) k_fft_test32(double* __restrict__ out, con
             uint32_t dw[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
-                dw[aa] = pack_f16x2((float)p[lane + 64 * aa], (float)p[lane + 64 * aa + M]);
+                // the coefficients of register aa as the bootstrap kernel reads them
+                const int j = K32_BLAYOUT ? bl_lane(lane) + bl_off(aa) : lane + 64 * aa;
+                dw[aa] = pack_f16x2((float)p[j], (float)p[j + M]);
                 tws[aa] = fwd_t2[lane * R1 + aa];
             }
-            fft512_fwd_mfma(v, dw, am, tws, tw, s_buf, lane);
+            fft512_fwd_mfma<K32_BLAYOUT>(v, dw, am, tws, tw, s_buf, lane);
         } else {
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
@@ -1845,17 +1898,10 @@ __device__ __forceinline__ void w2_rotate(uint32_t* u, const int32_t* s_acc, int
 // kernel, and the split two-wavefront kernel (w2_rotate_b's 128-word mirrors, G = 3, or w2_rotate's)
 constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4, W2_G_SPLIT2W = TFHE_W2_BLAYOUT ? 3 : W2_G_SPLIT;
 
-// Register r < 32 of lane (n0, q) = (lane & 15, lane >> 4) in the matrix-core operand layout:
-// coefficient w2b_lane(lane) + w2b_off(r), i.e. point j' = 16 g + n0 + 64 (2 q + t) of quarter
-// r >> 3 (j' + 512 (r >> 3)), rr = r & 7 = 2 g + t.  So prep's pairs (u[rr], u[rr + 16]) and
-// (u[rr + 8], u[rr + 24]) are the FP16 operands of fft512_fwd2w_mfma<true> as they stand.
-__host__ __device__ constexpr int w2b_off(int r) { return 16 * ((r & 7) >> 1) + 64 * (r & 1) + 512 * (r >> 3); }
-__device__ __forceinline__ int w2b_lane(int lane) { return (lane & 15) + 128 * (lane >> 4); }
-
-// w2_rotate for the operand layout: u[r] = ((X^a - 1) p_q + offset) ^ offset at coefficient
-// j = w2b_lane(lane) + w2b_off(r).  With d = j0 - a (j0 = w2b_lane(lane)), register r reads p at
-// e = (d + w2b_off(r)) mod 2N, negated when bit 11 of d + w2b_off(r) is set.  Quarter k = r >> 3
-// reads from one base b_k = (d + 512 k) mod N at offsets w2b_off(r & 7) <= 112, so 128-word mirrors
+// w2_rotate for the operand layout (bl_off): u[r] = ((X^a - 1) p_q + offset) ^ offset at
+// coefficient j = bl_lane(lane) + bl_off(r).  With d = j0 - a (j0 = bl_lane(lane)), register r reads p at
+// e = (d + bl_off(r)) mod 2N, negated when bit 11 of d + bl_off(r) is set.  Quarter k = r >> 3
+// reads from one base b_k = (d + 512 k) mod N at offsets bl_off(r & 7) <= 112, so 128-word mirrors
 // (w2_mirror<3>) cover every read, p0's through the shifted base as in w2_rotate.  The reads of
 // one register put the four rows of 16 lanes 128 words apart.
 template <int MW, bool ROT, uint32_t OFF>
@@ -1863,7 +1909,7 @@ __device__ __forceinline__ void w2_rotate_b(uint32_t* u, const int32_t* s_acc, i
     constexpr int N = W2_N, NC = 32;
     static_assert(MW >= 112, "mirrors cover the largest in-quarter offset");
     const int32_t* pq = s_acc + q * N;
-    const int j0 = w2b_lane(lane);
+    const int j0 = bl_lane(lane);
     if constexpr (ROT) {
         const int d = j0 - a;
         const uint32_t x = (uint32_t)d << 20;   // bit 11 of d + offset -> bit 31
@@ -1873,16 +1919,16 @@ __device__ __forceinline__ void w2_rotate_b(uint32_t* u, const int32_t* s_acc, i
         for (int k = 0; k < 4; ++k) base[k] = (d + 512 * k + sh) & (N - 1);
 #pragma unroll
         for (int r = 0; r < NC; ++r) {
-            const uint32_t pj = (uint32_t)pq[j0 + w2b_off(r)];
-            const uint32_t sv = (uint32_t)pq[base[r >> 3] + w2b_off(r & 7) - sh];
-            const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)w2b_off(r) << 20)) >> 31);
+            const uint32_t pj = (uint32_t)pq[j0 + bl_off(r)];
+            const uint32_t sv = (uint32_t)pq[base[r >> 3] + bl_off(r & 7) - sh];
+            const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)bl_off(r) << 20)) >> 31);
             uint32_t c = OFF - pj;
             asm("" : "+v"(c));
             u[r] = (((sv + m) ^ m) + c) ^ OFF;
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[j0 + w2b_off(r)] + OFF) ^ OFF;
+        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[j0 + bl_off(r)] + OFF) ^ OFF;
     }
 }
 
@@ -2600,7 +2646,7 @@ __global__ void __launch_bounds__(128) k_fft_test2w(double* __restrict__ out, co
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa) {
                 // the coefficients of register aa as the bootstrap kernel reads them
-                const int j = TFHE_W2_BLAYOUT ? w2b_lane(lane) + w2b_off(aa) : lane + 64 * aa;
+                const int j = TFHE_W2_BLAYOUT ? bl_lane(lane) + bl_off(aa) : lane + 64 * aa;
                 x0[aa] = pack_f16x2((float)p[j], (float)p[j + 1024]);
                 x1[aa] = pack_f16x2((float)p[j + 512], (float)p[j + 1536]);
             }

@@ -604,9 +604,13 @@ __device__ __forceinline__ void mfma_out_to_pass2(C2<float>* v, const f32x4_t* d
 // half of its first operand with the lower half of its second).  The outputs reach pass 2's layout
 // by one lane-bit exchange in registers (FWD_MFMA_X1_REGS) instead of exchange 1 through buf, which
 // is then used only by exchange 2.
+// IN_B: dw is already in the operand layout (lane (n0, q), register 2g + t: input 2q + t of vector
+// 16g + n0), as the bootstrap kernel reads it from the accumulator.
+template <bool IN_B = false>
 __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, const FwdMfmaA& am,
                                                 const C2<float>* fwd_t2, const DitTw<float>& tw, C2<float>* buf,
                                                 int lane) {
+    static_assert(!(IN_B && FWD_MFMA_K32), "the operand-layout input is for the K = 16 form");
     f32x4_t d[4];
 #if FWD_MFMA_K32
     // group g, lane (n0, q): the four inputs 4 (q & 1) + i of vector 16g + n0, from row g.  The swap
@@ -632,7 +636,7 @@ __device__ __forceinline__ void fft512_fwd_mfma(C2<float>* v, uint32_t* dw, cons
 #else
     // lane bits 4, 5 <-> register bits 1, 2 of dw: lane (n0, q) then holds inputs 2q, 2q + 1 of
     // vector 16g + n0 in dw[2g], dw[2g + 1]
-    swap_lane45_reg12(dw);
+    if constexpr (!IN_B) swap_lane45_reg12(dw);
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         const f16x4_t b = __builtin_bit_cast(f16x4_t, (uint2){dw[2 * g], dw[2 * g + 1]});
