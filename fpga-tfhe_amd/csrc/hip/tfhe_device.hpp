@@ -463,11 +463,25 @@ __device__ __forceinline__ void load_dit_tw(DitTw<T>& tw, const C2<T>* __restric
 // Exchange of lane bit 3 with a register bit for a pair of values (r0: register bit 0, r1: 1):
 // each lane sends its partner (lane ^ 8, DPP row_ror:8 within the row of 16) the value the partner
 // keeps and takes the partner's in its place.  s3: this lane's bit 3.
+#ifndef SWAP_BIT3_BANKMASK
+#define SWAP_BIT3_BANKMASK 1
+#endif
 __device__ __forceinline__ void swap_lane_bit3(uint32_t& r0, uint32_t& r1, bool s3) {
+#if SWAP_BIT3_BANKMASK
+    // Lanes with bit 3 set are DPP banks 2 and 3 of each row (bank_mask 0xC), the others banks 0
+    // and 1 (0x3); a DPP move writes only the lanes of its enabled banks and leaves the others at
+    // the old value.  So: r0 of the bit-3 lanes takes the partner's r1, then r1 of the other lanes
+    // takes the partner's original r0 (t).  No selects, since neither move needs one.
+    (void)s3;
+    const uint32_t t = r0;
+    r0 = (uint32_t)__builtin_amdgcn_update_dpp((int)r0, (int)r1, 0x128, 0xF, 0xC, false);
+    r1 = (uint32_t)__builtin_amdgcn_update_dpp((int)r1, (int)t, 0x128, 0xF, 0x3, false);
+#else
     const uint32_t send = s3 ? r0 : r1;
     const uint32_t got = (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xF, 0xF, true);  // row_ror:8
     r0 = s3 ? got : r0;
     r1 = s3 ? r1 : got;
+#endif
 }
 
 // 512-point transform with natural order in and out (lane L, register r <-> index L + 64 r), for
