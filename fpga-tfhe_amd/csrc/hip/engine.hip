@@ -529,6 +529,13 @@ constexpr int B64W_GATES = 8;
 constexpr int B64W_MAXN = 640;   // largest LWE dimension this form is launched for
 // the split kernel reads its digits in the matrix-core operand layout (lat_rotate_b) when its
 // forward transforms take the matrix-core first pass
+// 1: the split latency form reads its digits in the operand layout (lat_rotate_b) for its
+// matrix-core forward; 0 (default): lat_rotate and the lane swaps.  Off because a lone bootstrap's
+// transforms run back to back, so the operand layout's bank-conflicted reads sit on its critical
+// path: 2.95 against 2.85 ms per single gate.
+#ifndef LAT32_BLAYOUT
+#define LAT32_BLAYOUT 0
+#endif
 #ifndef K32_BLAYOUT
 #define K32_BLAYOUT (TFHE_FWD_MFMA && !FWD_MFMA_K32)
 #endif
@@ -980,8 +987,10 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
     static_assert(M == 512 && KPL >= 4, "the 512-point decimation-in-time transform; four output halves");
     constexpr uint32_t OFF = decomp_offset<L, BG>();
     constexpr uint32_t ROW_B = (uint32_t)(2 * M * sizeof(float4)), ROWSET_B = KPL * ROW_B;
-    constexpr int MW = lat_mirror<8>();
-    __shared__ int32_t s_acc_m[2 * N + 2 * MW];   // accumulator with mirrored ends (lat_rotate)
+    // accumulator with mirrored ends: 128 words for the operand-layout rotation (lat_rotate_b)
+    // that the matrix-core forward reads, 448 for lat_rotate
+    constexpr int MG = LAT32_BLAYOUT ? 3 : 8, MW = lat_mirror<MG>();
+    __shared__ int32_t s_acc_m[2 * N + 2 * MW];
     int32_t* const s_acc = s_acc_m + MW;
     __shared__ int16_t s_bara[kMaxLweN];
     __shared__ C2<float> s_x[KPL][xbuf_elems<M>()];
@@ -991,7 +1000,7 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
     const int g = blockIdx.x;
     if (w == 0) {
         boot_prologue<N>(A, g, lane, s_acc, s_bara);
-        lat_mirror_fill<8>(s_acc, lane);
+        lat_mirror_fill<MG>(s_acc, lane);
     }
     __syncthreads();
 
@@ -1029,14 +1038,18 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
         if (loaded != i) load_row(i);
         {
             uint32_t u[NC];
-            lat_rotate<8, OFF>(u, s_acc, q, a, lane);
+#if LAT32_BLAYOUT
+            lat_rotate_b<MW, OFF>(u, s_acc, q, a, lane);
+#else
+            lat_rotate<MG, OFF>(u, s_acc, q, a, lane);
+#endif
             C2<float> v[R1];
 #if TFHE_LAT_FWD_MFMA
             uint32_t dw[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
                 dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
-            fft512_fwd_mfma(v, dw, am, tws, tw, s_x[w], lane);
+            fft512_fwd_mfma<LAT32_BLAYOUT>(v, dw, am, tws, tw, s_x[w], lane);
 #else
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
@@ -1075,9 +1088,9 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
                 const int32_t dl = (int32_t)(round_mod32(z.x) << sh), dh = (int32_t)(round_mod32(z.y) << sh);
                 atomicAdd(acc + jj, dl);
                 atomicAdd(acc + jj + M, dh);
-                // the mirrors take the same increments: p0 from N - 448 up, p1 below 448
-                if (j == 0 && aa >= 1) atomicAdd(acc + jj + M - N, dh);
-                if (j == 1 && aa <= 6) atomicAdd(acc + jj + N, dl);
+                // the mirrors take the same increments: p0 from N - MW up, p1 below MW
+                if (j == 0 && aa >= 8 - MW / 64) atomicAdd(acc + jj + M - N, dh);
+                if (j == 1 && aa < MW / 64) atomicAdd(acc + jj + N, dl);
             }
         }
         __syncthreads();
