@@ -56,6 +56,12 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #endif
 // N = 2048 split kernel with the matrix-core forward: the rotation reads each lane's coefficients
 // in the matrix-core operand layout (w2_rotate_b), so no lane exchange precedes the matrix products.
+// 1: the split kernels' operand-layout rotations leave x + offset and the digits go to FP16 by byte
+// permutes and one packed FP16 add (digits8_f16x2, digits7_f16x2); 0: signed bit-field extracts,
+// int-to-float conversions and a packing conversion.
+#ifndef DIGIT_PERM
+#define DIGIT_PERM 1
+#endif
 #ifndef TFHE_W2_BLAYOUT
 #define TFHE_W2_BLAYOUT TFHE_W2_FWD_MFMA
 #endif
@@ -388,7 +394,7 @@ __device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, in
 // mod 2N, d = bl_lane(lane) - a, negated when bit 10 of d + bl_off(r) is set.  Each half r >> 3
 // reads from one base at offsets bl_off(r & 7) <= 112, so 128-word mirrors (lat_mirror<3>) cover
 // every read.  The reads of one register put the four rows of 16 lanes 128 words apart.
-template <int MW, uint32_t OFF>
+template <int MW, uint32_t OFF, bool XOFF = true>
 __device__ __forceinline__ void lat_rotate_b(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
     constexpr int N = 1024, NC = 16;
     static_assert(MW >= 112, "mirrors cover the largest in-half offset");
@@ -407,7 +413,8 @@ __device__ __forceinline__ void lat_rotate_b(uint32_t* u, const int32_t* s_acc, 
         const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)bl_off(r) << 21)) >> 31);
         uint32_t c = OFF - pj;
         asm("" : "+v"(c));
-        u[r] = (((sv + m) ^ m) + c) ^ OFF;
+        u[r] = ((sv + m) ^ m) + c;
+        if constexpr (XOFF) u[r] ^= OFF;
     }
 }
 // After coefficients j and j + N/2 of p_q (j = lane + 64 aa, aa < 8, the index within p_q) have been
@@ -748,11 +755,18 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
         uint32_t u[NC];
 #if K32_BLAYOUT
         if constexpr (ROT) {
-            lat_rotate_b<lat_mirror<K32_MIRROR_G>(), OFF>(u, s_acc, q, a, lane);
+            lat_rotate_b<lat_mirror<K32_MIRROR_G>(), OFF, !DIGIT_PERM>(u, s_acc, q, a, lane);
         } else {
 #pragma unroll
-            for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)s_acc[q * N + bl_lane(lane) + bl_off(r)] + OFF) ^ OFF;
+            for (int r = 0; r < NC; ++r)
+                u[r] = ((uint32_t)s_acc[q * N + bl_lane(lane) + bl_off(r)] + OFF) ^ (DIGIT_PERM ? 0u : OFF);
         }
+#if DIGIT_PERM
+        // the digit fields of each register pair, byte-permuted once for the three levels
+        static_assert(BG == 7 && L == 3, "the byte layout of digits7_f16x2");
+#pragma unroll
+        for (int aa = 0; aa < R1; ++aa) digits7_prep(u[aa], u[aa + R1], u[aa], u[aa + R1]);
+#endif
 #else
         if constexpr (ROT) {
             lat_rotate<K32_MIRROR_G, OFF>(u, s_acc, q, a, lane);
@@ -769,7 +783,11 @@ __device__ __forceinline__ void step32(int32_t* s_acc, C2<float>* s_buf, int a, 
             uint32_t dw[R1];
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
+#if K32_BLAYOUT && DIGIT_PERM
+                dw[aa] = digits7_f16x2(u[aa], u[aa + R1], lvl);
+#else
                 dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
+#endif
             fft512_fwd_mfma<K32_BLAYOUT>(v, dw, am, tws, tw, s_buf, lane);
 #else
 #pragma unroll
@@ -1922,7 +1940,7 @@ constexpr int W2_G_SPLIT = 8, W2_G_FP64 = 4, W2_G_SPLIT2W = TFHE_W2_BLAYOUT ? 3 
 // reads from one base b_k = (d + 512 k) mod N at offsets bl_off(r & 7) <= 112, so 128-word mirrors
 // (w2_mirror<3>) cover every read, p0's through the shifted base as in w2_rotate.  The reads of
 // one register put the four rows of 16 lanes 128 words apart.
-template <int MW, bool ROT, uint32_t OFF>
+template <int MW, bool ROT, uint32_t OFF, bool XOFF = true>
 __device__ __forceinline__ void w2_rotate_b(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
     constexpr int N = W2_N, NC = 32;
     static_assert(MW >= 112, "mirrors cover the largest in-quarter offset");
@@ -1942,11 +1960,12 @@ __device__ __forceinline__ void w2_rotate_b(uint32_t* u, const int32_t* s_acc, i
             const uint32_t m = (uint32_t)((int32_t)(x + ((uint32_t)bl_off(r) << 20)) >> 31);
             uint32_t c = OFF - pj;
             asm("" : "+v"(c));
-            u[r] = (((sv + m) ^ m) + c) ^ OFF;
+            u[r] = ((sv + m) ^ m) + c;
+            if constexpr (XOFF) u[r] ^= OFF;
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[j0 + bl_off(r)] + OFF) ^ OFF;
+        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[j0 + bl_off(r)] + OFF) ^ (XOFF ? OFF : 0u);
     }
 }
 
@@ -1974,7 +1993,7 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     if (work) {
     uint32_t u[NC];
 #if TFHE_W2_BLAYOUT
-    auto rotate = [&](int q) { w2_rotate_b<w2_mirror<W2_G_SPLIT2W>(), ROT, OFF>(u, s_acc, q, a, lane); };
+    auto rotate = [&](int q) { w2_rotate_b<w2_mirror<W2_G_SPLIT2W>(), ROT, OFF, !DIGIT_PERM>(u, s_acc, q, a, lane); };
 #else
     auto rotate = [&](int q) { w2_rotate<W2_G_SPLIT2W, ROT, OFF>(u, s_acc, q, a, lane); };
 #endif
@@ -1985,8 +2004,14 @@ __device__ __forceinline__ void step2w32(int32_t* s_acc, void* sx_own, const voi
     auto prep = [&](uint32_t* x0, uint32_t* x1, int lvl) {
 #pragma unroll
         for (int aa = 0; aa < R1; ++aa) {
+#if TFHE_W2_BLAYOUT && DIGIT_PERM
+            static_assert(BG == 8, "byte-aligned digits");
+            x0[aa] = digits8_f16x2(u[aa], u[aa + 16], lvl);
+            x1[aa] = digits8_f16x2(u[aa + 8], u[aa + 24], lvl);
+#else
             x0[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + 16], lvl));
             x1[aa] = pack_f16x2((float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl));
+#endif
         }
     };
 #else

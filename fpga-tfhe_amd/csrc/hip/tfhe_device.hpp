@@ -574,6 +574,35 @@ __device__ __forceinline__ uint32_t pack_f16x2(float x, float y) {
     return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x, y));
 }
 
+// Gadget digits straight to FP16 pairs, from y = x + offset (no xor): each Bg-bit field of y is
+// f = digit + 2^(Bg-1), unsigned.  The field goes into the mantissa of an FP16 number whose
+// exponent makes one unit of f one unit of value, and one packed FP16 add subtracts the exponent's
+// base and 2^(Bg-1), exactly (every value is an integer below 2^11).  Replaces a signed bit-field
+// extract and an int-to-float conversion per digit and a packing conversion per pair.
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f16x2_add(uint32_t w, _Float16 c) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(f16x2_t, w) + f16x2_t{c, c});
+}
+// Bg = 8: level lvl is byte 3 - lvl of a (low half) and b (high half); with 0x64 (1024 + f) above
+// each byte, minus 1024 + 128.
+__device__ __forceinline__ uint32_t digits8_f16x2(uint32_t a, uint32_t b, int lvl) {
+    const uint32_t k = 3u - (uint32_t)lvl;
+    const uint32_t w = __builtin_amdgcn_perm(b, a, k | (0x0Cu << 8) | ((4u + k) << 16) | (0x0Cu << 24));
+    return f16x2_add(w | 0x64006400u, (_Float16)-1152.0f);
+}
+// Bg = 7: the top two bytes of a and b (p0 = byte-permuted (a >> 16, b >> 16)) hold the fields of
+// levels 0 (bits 9-15 of each half) and 1 (bits 2-8), bytes 1-2 (p1) the field of level 2 (bits
+// 3-9).  Each is moved to mantissa bits 3-9 of exponent 0x58 (128 + f, steps of 1/8) by a 32-bit
+// shift, whose bits crossing between the halves the mask drops, then 128 + 64 is subtracted.
+__device__ __forceinline__ void digits7_prep(uint32_t a, uint32_t b, uint32_t& p0, uint32_t& p1) {
+    p0 = __builtin_amdgcn_perm(b, a, 0x07060302u);
+    p1 = __builtin_amdgcn_perm(b, a, 0x06050201u);
+}
+__device__ __forceinline__ uint32_t digits7_f16x2(uint32_t p0, uint32_t p1, int lvl) {
+    const uint32_t h = lvl == 0 ? p0 >> 6 : lvl == 1 ? p0 << 1 : p1;
+    return f16x2_add((h & 0x03F803F8u) | 0x58005800u, (_Float16)-192.0f);
+}
+
 // Lane bits 4 and 5 trade places with register bits 1 and 2 of dw[8] (v_permlane16_swap of
 // registers (r, r + 2), then v_permlane32_swap of (r, r + 4)).
 __device__ __forceinline__ void swap_lane45_reg12(uint32_t* dw) {
