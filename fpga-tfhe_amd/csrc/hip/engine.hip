@@ -370,7 +370,7 @@ __device__ __forceinline__ void lat_mirror_fill(int32_t* s_acc, int lane) {
         s_acc[2 * N + k] = s_acc[N + k];
     }
 }
-template <int G, uint32_t OFF>
+template <int G, uint32_t OFF, bool XOFF = true>
 __device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
     constexpr int N = 1024, NC = 16, MW = lat_mirror<G>(), NG = NC / G;
     const int32_t* pq = s_acc + q * N;
@@ -387,7 +387,8 @@ __device__ __forceinline__ void lat_rotate(uint32_t* u, const int32_t* s_acc, in
         const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
         uint32_t c = OFF - pj;
         asm("" : "+v"(c));
-        u[r] = (((sv + m) ^ m) + c) ^ OFF;
+        u[r] = ((sv + m) ^ m) + c;
+        if constexpr (XOFF) u[r] ^= OFF;
     }
 }
 // lat_rotate for the operand layout (bl_off, r < 16): register r reads p at e = (d + bl_off(r))
@@ -1056,17 +1057,26 @@ __global__ void __launch_bounds__(64 * 2 * L) k_boot32lat(BootKArgs A) {
         if (loaded != i) load_row(i);
         {
             uint32_t u[NC];
+            // with the matrix-core forward and DIGIT_PERM the rotation leaves x + offset for
+            // digits7_f16x2 (the pairs (u[aa], u[aa + R1]) are the same in either layout)
+            constexpr bool XO = !(TFHE_LAT_FWD_MFMA && DIGIT_PERM);
 #if LAT32_BLAYOUT
-            lat_rotate_b<MW, OFF>(u, s_acc, q, a, lane);
+            lat_rotate_b<MW, OFF, XO>(u, s_acc, q, a, lane);
 #else
-            lat_rotate<MG, OFF>(u, s_acc, q, a, lane);
+            lat_rotate<MG, OFF, XO>(u, s_acc, q, a, lane);
 #endif
             C2<float> v[R1];
 #if TFHE_LAT_FWD_MFMA
             uint32_t dw[R1];
+#if DIGIT_PERM
+            static_assert(BG == 7 && L == 3, "the byte layout of digits7_f16x2");
+#pragma unroll
+            for (int aa = 0; aa < R1; ++aa) dw[aa] = digits7_f16x2_one(u[aa], u[aa + R1], lvl);
+#else
 #pragma unroll
             for (int aa = 0; aa < R1; ++aa)
                 dw[aa] = pack_f16x2((float)digit_sx<BG>(u[aa], lvl), (float)digit_sx<BG>(u[aa + R1], lvl));
+#endif
             fft512_fwd_mfma<LAT32_BLAYOUT>(v, dw, am, tws, tw, s_x[w], lane);
 #else
 #pragma unroll

@@ -602,6 +602,12 @@ __device__ __forceinline__ uint32_t digits7_f16x2(uint32_t p0, uint32_t p1, int 
     const uint32_t h = lvl == 0 ? p0 >> 6 : lvl == 1 ? p0 << 1 : p1;
     return f16x2_add((h & 0x03F803F8u) | 0x58005800u, (_Float16)-192.0f);
 }
+// The same for one level only, with the one permute that level needs.
+__device__ __forceinline__ uint32_t digits7_f16x2_one(uint32_t a, uint32_t b, int lvl) {
+    const uint32_t p = __builtin_amdgcn_perm(b, a, lvl < 2 ? 0x07060302u : 0x06050201u);
+    const uint32_t h = lvl == 0 ? p >> 6 : lvl == 1 ? p << 1 : p;
+    return f16x2_add((h & 0x03F803F8u) | 0x58005800u, (_Float16)-192.0f);
+}
 
 // Lane bits 4 and 5 trade places with register bits 1 and 2 of dw[8] (v_permlane16_swap of
 // registers (r, r + 2), then v_permlane32_swap of (r, r + 4)).
