@@ -56,6 +56,12 @@ constexpr int KS_PAIR_ROWS = 15;  // keyswitch key rows per digit pair: (d1, d2)
 #endif
 // N = 2048 split kernel with the matrix-core forward: the rotation reads each lane's coefficients
 // in the matrix-core operand layout (w2_rotate_b), so no lane exchange precedes the matrix products.
+// N = 2048 split latency form (k_boot2wlat): forward transforms with the matrix-core first pass (in
+// the conventional layout, with the lane swaps) and digits by byte permutes; inverse transforms with
+// the pass-1 table in pass 2's layout, as k_boot2w
+#ifndef TFHE_W2LAT_FWD_MFMA
+#define TFHE_W2LAT_FWD_MFMA 1
+#endif
 // 1: the split kernels' operand-layout rotations leave x + offset and the digits go to FP16 by byte
 // permutes and one packed FP16 add (digits8_f16x2, digits7_f16x2); 0: signed bit-field extracts,
 // int-to-float conversions and a packing conversion.
@@ -1912,7 +1918,7 @@ __device__ __forceinline__ void w2_mirror_update(int32_t* s_acc, int q, int wave
 // wrong results) and no read needs wrap arithmetic.  +-p_e = (p_e + m) ^ m for m = 0 or -1, and the
 // opaque offset term keeps the compiler from reassociating the sum, so that the xor and the add
 // fuse into one v_xad_u32: four instructions per coefficient and one sign extract per register.
-template <int G, bool ROT, uint32_t OFF>
+template <int G, bool ROT, uint32_t OFF, bool XOFF = true>
 __device__ __forceinline__ void w2_rotate(uint32_t* u, const int32_t* s_acc, int q, int a, int lane) {
     constexpr int N = W2_N, NC = 32, MW = w2_mirror<G>(), NG = NC / G;
     const int32_t* pq = s_acc + q * N;
@@ -1932,11 +1938,12 @@ __device__ __forceinline__ void w2_rotate(uint32_t* u, const int32_t* s_acc, int
             const uint32_t m = (uint32_t)((int32_t)(sgn << (31 - r)) >> 31);
             uint32_t c = OFF - pj;
             asm("" : "+v"(c));
-            u[r] = (((sv + m) ^ m) + c) ^ OFF;
+            u[r] = ((sv + m) ^ m) + c;
+            if constexpr (XOFF) u[r] ^= OFF;
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[lane + 64 * r] + OFF) ^ OFF;
+        for (int r = 0; r < NC; ++r) u[r] = ((uint32_t)pq[lane + 64 * r] + OFF) ^ (XOFF ? OFF : 0u);
     }
 }
 
@@ -2409,8 +2416,17 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
     constexpr size_t ROWSET = (size_t)KPL * 2 * M;
     using TW = typename std::conditional<P == 1, LaneTw<float, 8>, LaneTwS<double, 8>>::type;
     TW tw;
+    [[maybe_unused]] f16x8_t a2[2];
     if constexpr (P == 1) {
+#if TFHE_W2LAT_FWD_MFMA
+        // the pass-1 twiddles in pass 2's layout (forward and inverse) and this lane's entries of the
+        // two FP16 pieces of parity v's pass-1 matrix
+        load_lane_tw(tw, (const C2<float>*)A.fwd_t2 + v * 64 * 8, (const C2<float>*)A.t2, lane);
+        a2[0] = ((const f16x8_t*)A.fwd_a16)[(v * 64 + lane) * 2];
+        a2[1] = ((const f16x8_t*)A.fwd_a16)[(v * 64 + lane) * 2 + 1];
+#else
         load_lane_tw(tw, (const C2<float>*)A.t1 + v * 64 * 8, (const C2<float>*)A.t2, lane);
+#endif
     } else {
 #pragma unroll
         for (int r = 0; r < 8; ++r) tw.t1[r] = ((const C2<double>*)A.t1)[(v * 64 + lane) * 8 + r];
@@ -2442,6 +2458,27 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                 // 1 is rotated and its digits prepared while row (0, lvl)'s second polynomial is in
                 // flight
                 uint32_t u[NC];
+#if TFHE_W2LAT_FWD_MFMA
+                // x[aa] carries the FP16 digit pairs (P0_aa, P1_aa) until its transform
+                (void)hc;
+                (void)rw32;
+                auto rotate = [&](int q) { w2_rotate<G, true, OFF, false>(u, s_acc, q, a, lane); };
+                auto prep = [&](C2<float>* x) {
+#pragma unroll
+                    for (int aa = 0; aa < R1; ++aa)
+                        x[aa] = C2<float>{__uint_as_float(digits8_f16x2(u[aa], u[aa + 16], lvl)),
+                                          __uint_as_float(digits8_f16x2(u[aa + 8], u[aa + 24], lvl))};
+                };
+                auto fwd = [&](C2<float>* x) {
+                    uint32_t p0[R1], p1[R1];
+#pragma unroll
+                    for (int aa = 0; aa < R1; ++aa) {
+                        p0[aa] = __float_as_uint(x[aa].x);
+                        p1[aa] = __float_as_uint(x[aa].y);
+                    }
+                    fft512_fwd2w_mfma<false>(x, p0, p1, a2, twl.t1, twl.t2, (C2<float>*)s_x[w], lane);
+                };
+#else
                 auto rotate = [&](int q) { w2_rotate<G, true, OFF>(u, s_acc, q, a, lane); };
                 auto prep = [&](C2<float>* x) {
 #pragma unroll
@@ -2450,6 +2487,8 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                                           (float)digit_sx<BG>(u[aa + 8], lvl), (float)digit_sx<BG>(u[aa + 24], lvl),
                                           hc, rw32[aa], aa == 0);
                 };
+                auto fwd = [&](C2<float>* x) { fft_fwd<R1>(x, twl, s_x[w], lane); };
+#endif
                 C2<float> x[R1];
                 rotate(0);
                 prep(x);
@@ -2460,7 +2499,7 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
                     float4 k0[R1];
 #pragma unroll
                     for (int r = 0; r < R1; ++r) k0[r] = kp[r * 64];
-                    fft_fwd<R1>(x, twl, s_x[w], lane);
+                    fwd(x);
 #pragma unroll
                     for (int r = 0; r < R1; ++r) {
                         cmac(acc[0][r], x[r], C2<float>{k0[r].x, k0[r].y});
@@ -2564,7 +2603,11 @@ __global__ void __launch_bounds__(64 * (P ? 8 : 2 * L)) k_boot2wlat(BootKArgs A)
             }
             C2<double>* xo = (C2<double>*)&s_part[v][k][0][0];
             if constexpr (P == 1) {
+#if TFHE_W2LAT_FWD_MFMA
+                fft512_inv2w_t1p(o, twl.t1, twl.t2, (C2<float>*)s_x[w], lane);
+#else
                 fft_inv<R1>(o, twl, s_x[w], lane);
+#endif
 #pragma unroll
                 for (int aa = 0; aa < R1; ++aa) {
                     own[aa] = cmulc(C2<double>{(double)o[aa].x, (double)o[aa].y}, rws[aa]);
