@@ -1149,10 +1149,19 @@ constexpr int KS_GATES = 256;
 #ifndef TFHE_KS_C
 #define TFHE_KS_C 32   // output coefficients per workgroup (registers per lane)
 #endif
-constexpr int KS_IBLK = 16;
+// Occupancy: 8 input indices per staging block (26 KB of LDS per workgroup at t = 8) and registers
+// for 6 wavefronts per SIMD let six workgroups share a CU, so a 16384-ciphertext launch (1280
+// workgroups) is resident at once instead of running 1.25 rounds of four per CU.
+#ifndef TFHE_KS_IBLK
+#define TFHE_KS_IBLK 8    // input indices per staging block of the extracted samples
+#endif
+constexpr int KS_IBLK = TFHE_KS_IBLK;
+#ifndef TFHE_KS_WPE
+#define TFHE_KS_WPE 6     // wavefronts per SIMD the register allocation must allow
+#endif
 
 template <int TKS, int KN, int NP, int C>
-__global__ void __launch_bounds__(KS_GATES) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
+__global__ void __launch_bounds__(KS_GATES) __attribute__((amdgpu_waves_per_eu(TFHE_KS_WPE))) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
                                                         int32_t add_b, const int32_t* __restrict__ ksk,
                                                         int32_t* __restrict__ out, int count, int n,
                                                         int32_t* const* __restrict__ out_tab, int out_seg) {
