@@ -158,9 +158,12 @@ hipError_t launch_extract(int N, int32_t* u, const int32_t* acc, int count, hipS
 hipError_t launch_rotate_decompose(const DevParams& p, int32_t* rot, int32_t* digits, const int32_t* acc,
                                    const int32_t* bara, int count, hipStream_t st);
 // inverse: 0 forward, 1 inverse, 2 the split path's forward of digit inputs with its first pass on
-// the matrix cores (N = 1024, split FP32 only)
+// the matrix cores (split FP32 only)
 hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse, double* out, const void* in,
                            int count, hipStream_t st);
+// the split kernels' FP16 digit conversion (digits7_f16x2 / digits7_f16x2_one at bg = 7, variant 0
+// / 1; digits8_f16x2 at bg = 8) of coefficient pairs x[2i], x[2i + 1]: out[i][lvl] = the two digits
+hipError_t launch_digits_test(int bg, int variant, float* out, const int32_t* x, int count, hipStream_t st);
 hipError_t launch_valu_probe(int fp64, void* out, int blocks, int iters, hipStream_t st);
 hipError_t launch_decrypt(int32_t* bits, const int32_t* samples, const int32_t* key, int n, int count,
                           hipStream_t st);

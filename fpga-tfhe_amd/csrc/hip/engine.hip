@@ -3577,6 +3577,37 @@ hipError_t launch_fft_test(const DevTwiddles& tw, int N, Prec prec, int inverse,
     return hipGetLastError();
 }
 
+// The split kernels' digit conversion on its own: pair i is (x[2i], x[2i + 1]), taken as the
+// rotation leaves them (x + offset, no xor); out[(i * 3 + lvl) * 2 + h] = digit lvl of x[2i + h].
+template <int BG>
+__global__ void __launch_bounds__(256) k_digits_test(float* __restrict__ out, const int32_t* __restrict__ x,
+                                                     int variant, int count) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    constexpr uint32_t OFF = decomp_offset<3, BG>();
+    const uint32_t a = (uint32_t)x[2 * i] + OFF, b = (uint32_t)x[2 * i + 1] + OFF;
+    uint32_t p0 = 0, p1 = 0;
+    if constexpr (BG == 7) digits7_prep(a, b, p0, p1);
+#pragma unroll
+    for (int lvl = 0; lvl < 3; ++lvl) {
+        uint32_t w;
+        if constexpr (BG == 7) w = variant ? digits7_f16x2_one(a, b, lvl) : digits7_f16x2(p0, p1, lvl);
+        else w = digits8_f16x2(a, b, lvl);
+        const f16x2_t h = __builtin_bit_cast(f16x2_t, w);
+        out[(i * 3 + lvl) * 2] = (float)h[0];
+        out[(i * 3 + lvl) * 2 + 1] = (float)h[1];
+    }
+}
+
+hipError_t launch_digits_test(int bg, int variant, float* out, const int32_t* x, int count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    const dim3 grid((count + 255) / 256);
+    if (bg == 7) hipLaunchKernelGGL(k_digits_test<7>, grid, dim3(256), 0, st, out, x, variant, count);
+    else if (bg == 8) hipLaunchKernelGGL(k_digits_test<8>, grid, dim3(256), 0, st, out, x, variant, count);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_valu_probe(int fp64, void* out, int blocks, int iters, hipStream_t st) {
     if (fp64)
         hipLaunchKernelGGL((k_valu_probe<double>), dim3(blocks), dim3(256), 0, st, (double*)out, 1.0000001, 1e-9, iters);

@@ -983,6 +983,24 @@ extern "C" int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, 
 extern "C" int tfhe_gpu_fft_forward_digits_batch(int32_t N, double* out, const int32_t* in, int32_t count) {
     return fft_stage(N, TFHE_PREC_SPLIT32, 2, out, in, count);
 }
+extern "C" int tfhe_gpu_split_digits_batch(int32_t Bgbit, int32_t variant, float* out, const int32_t* x,
+                                           int32_t count) {
+    Ctx& cx = ctx();
+    std::lock_guard<std::recursive_mutex> g(cx.mu);
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (count < 0) return arg_fail("negative count");
+    if (count == 0) return TFHE_OK;
+    if (!out || !x) return arg_fail("null argument");
+    if (Bgbit != 7 && Bgbit != 8) return arg_fail("Bgbit must be 7 or 8 (l = 3)");
+    if (variant != 0 && variant != 1) return arg_fail("variant must be 0 or 1");
+    const size_t ib = (size_t)count * 2 * 4, ob = (size_t)count * 6 * 4;
+    void *di, *dout;
+    if ((rc = scratch().get(0, ib, &di)) || (rc = scratch().get(1, ob, &dout)) || (rc = h2d(di, x, ib))) return rc;
+    hipError_t e = launch_digits_test(Bgbit, variant, (float*)dout, (const int32_t*)di, count, cx.stream);
+    if (e != hipSuccess) return hip_fail("digits launch", e);
+    return d2h_sync(out, dout, ob);
+}
 
 // ---- device memory helpers ----
 extern "C" int tfhe_gpu_malloc(void** ptr, size_t bytes) {

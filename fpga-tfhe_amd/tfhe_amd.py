@@ -123,6 +123,7 @@ def lib():
         "tfhe_gpu_rotate_decompose_batch": (ctypes.c_int, [P(FlatParams), _i32p, _i32p, _i32p, _i32p, i32]),
         "tfhe_gpu_fft_forward_batch": (ctypes.c_int, [i32, i32, _f64p, _i32p, i32]),
         "tfhe_gpu_fft_forward_digits_batch": (ctypes.c_int, [i32, _f64p, _i32p, i32]),
+        "tfhe_gpu_split_digits_batch": (ctypes.c_int, [i32, i32, P(ctypes.c_float), _i32p, i32]),
         "tfhe_gpu_fft_inverse_batch": (ctypes.c_int, [i32, i32, _f64p, _f64p, i32]),
         "tfhe_gpu_fft_q4_batch": (ctypes.c_int, [i32, _f64p, _vp, i32]),
         "tfhe_gpu_malloc": (ctypes.c_int, [P(_vp), ctypes.c_size_t]),
@@ -419,6 +420,17 @@ def fft_forward(polys, prec=PREC_FP64):
     out = np.zeros((cnt, N), np.float64)
     _ck(lib().tfhe_gpu_fft_forward_batch(N, prec, f64p(out), i32p(polys), cnt), "fft_forward")
     return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+def split_digits(x, Bgbit, variant=0):
+    """The split kernels' gadget digits of coefficient pairs as FP16 (returned as float32):
+    x [count][2] int32 -> [count][3][2], digit lvl of each coefficient of the pair (l = 3)."""
+    x = np.ascontiguousarray(x, np.int32)
+    cnt = x.shape[0]
+    out = np.zeros((cnt, 3, 2), np.float32)
+    _ck(lib().tfhe_gpu_split_digits_batch(Bgbit, variant, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           i32p(x), cnt), "split_digits")
+    return out
 
 
 def fft_forward_digits(polys):

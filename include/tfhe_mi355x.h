@@ -451,6 +451,12 @@ int tfhe_gpu_fft_inverse_batch(int32_t N, int32_t prec, double* out, const doubl
  * as tfhe_gpu_fft_forward_batch.  No counterpart in the reference (a check of the device
  * transform). */
 int tfhe_gpu_fft_forward_digits_batch(int32_t N, double* out, const int32_t* in, int32_t count);
+/* The split-FP32 kernels' conversion of gadget digits (l = 3, Bgbit 7 or 8) to FP16 pairs, on its
+ * own: coefficient pairs x [count][2] -> out [count][3][2], out[i][lvl][h] = digit lvl of x[i][h]
+ * (tGswTorus32PolynomialDecompH.cpp:29-44 semantics).  variant (Bgbit 7 only): 0 the batched
+ * kernel's three-level form, 1 the latency form's one-level form.  No counterpart in the reference
+ * (a check of the device digit conversion). */
+int tfhe_gpu_split_digits_batch(int32_t Bgbit, int32_t variant, float* out, const int32_t* x, int32_t count);
 /* The same two transforms (N = 1024, FP32) in the layout of launch form 6, four polynomials per
  * wavefront with 16 lanes each: forward int32 [count][1024] -> F natural order (interleaved
  * doubles [count][1024]) when inverse = 0, else F -> integer-unit doubles [count][1024]. */

@@ -440,3 +440,17 @@ def test_mux_xor_8192_full_workload(env2048, prec):
     finally:
         for b in bufs + outs:
             b.free()
+
+
+def test_split_digits_fp16_n2048_bit_exact():
+    """The N = 2048 split kernels' conversion of its 8-bit gadget digits to FP16 pairs (one byte
+    permute, an or and one packed FP16 add) against the oracle's signed decomposition."""
+    rng = np.random.default_rng(2052)
+    x = np.concatenate([rand_torus(rng, 4096),
+                        np.array([0, -1, 2**31 - 1, -2**31, 0x7F7F7F00, -0x7F7F7F00, 0x80808000 - 2**32, 0x7FFFFF],
+                                 np.int64).astype(np.int32)])
+    x = np.concatenate([x, x[::-1]]).reshape(-1, 2)
+    got = T().split_digits(x, P.Bgbit)
+    for h in range(2):
+        want = O.decompose(x[:, h], P.l, P.Bgbit)
+        assert np.array_equal(got[:, :, h].T, want.astype(np.float32)), h

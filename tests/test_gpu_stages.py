@@ -254,3 +254,20 @@ def test_split_hi_rounding_margin(gpu_env):
           flush=True)
     assert dist.max() < 0.25
     assert worst_exact < 0.25
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_split_digits_fp16_bit_exact(rng, variant):
+    """The split kernels' conversion of gadget digits to FP16 pairs (byte permutes, an exponent
+    trick and one packed FP16 add) against the oracle's signed decomposition, digit by digit:
+    random coefficients and the ends of each digit's range (every 7-bit field at -64 and 63, the
+    coefficients whose offset addition carries through every field).  variant 0 is the batched
+    kernel's three-level form, 1 the latency form's one-level form."""
+    x = np.concatenate([rand_torus(rng, 4096),
+                        np.array([0, -1, 2**31 - 1, -2**31, 0x7E000000, -0x7E000000, 0x3F7F0000, -0x40810000],
+                                 np.int64).astype(np.int32)])
+    x = np.concatenate([x, x[::-1]]).reshape(-1, 2)
+    got = T().split_digits(x, P.Bgbit, variant)
+    for h in range(2):
+        want = O.decompose(x[:, h], P.l, P.Bgbit)      # [l][count]
+        assert np.array_equal(got[:, :, h].T, want.astype(np.float32)), h
