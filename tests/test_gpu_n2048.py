@@ -454,3 +454,25 @@ def test_split_digits_fp16_n2048_bit_exact():
     for h in range(2):
         want = O.decompose(x[:, h], P.l, P.Bgbit)
         assert np.array_equal(got[:, :, h].T, want.astype(np.float32)), h
+
+
+def test_extern_product_n2048_fp64_exact_sweep(env2048):
+    """The N = 2048 FP64 external product bit-exact with the exact int64 product over 32 products
+    (rows spread over the key; random, extreme-digit and int32-end accumulators)."""
+    _, dk, keys = env2048
+    rng = np.random.default_rng(2053)
+    unit = sum(1 << (32 - (p + 1) * P.Bgbit) for p in range(P.l))
+    lo, hi = -(1 << (P.Bgbit - 1)) * unit, ((1 << (P.Bgbit - 1)) - 1) * unit
+    alt = np.where(np.arange(N) % 2 == 0, lo, hi)
+    special = np.stack([np.full((2, N), lo), np.full((2, N), hi), np.stack([alt, alt[::-1]]),
+                        np.full((2, N), 2**31 - 1), np.full((2, N), -2**31)]).astype(np.int64)
+    acc = np.concatenate([rand_torus(rng, (27, 2, N)), ((special + 2**31) % 2**32 - 2**31).astype(np.int32)])
+    rows = np.linspace(0, P.n - 1, acc.shape[0]).astype(np.int32)
+    got, raw = dk.extern_product(acc, rows, prec=0, want_raw=True)
+    worst = 0.0
+    for g in range(acc.shape[0]):
+        want, exact = O.extern_product_exact(P, acc[g], keys.bk[rows[g]], want_raw=True)
+        assert np.array_equal(got[g], want), (g, int(rows[g]))
+        worst = max(worst, float(np.abs(raw[g] - exact.astype(np.float64)).max()))
+    print(f"N=2048 FP64 external product sweep ({acc.shape[0]} products): max |pre-rounding - exact| = {worst:.3e}")
+    assert worst < 0.125

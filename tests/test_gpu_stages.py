@@ -271,3 +271,36 @@ def test_split_digits_fp16_bit_exact(rng, variant):
     for h in range(2):
         want = O.decompose(x[:, h], P.l, P.Bgbit)      # [l][count]
         assert np.array_equal(got[:, :, h].T, want.astype(np.float32)), h
+
+
+def _extreme_accumulators(P_, n_rand, rng):
+    """Accumulators for the exactness sweeps: random torus values, every coefficient with all three
+    digits at -2^(Bg-1) or at 2^(Bg-1) - 1 (the largest digit magnitudes the products see), the two
+    alternating, and the int32 ends."""
+    Np = P_.N
+    unit = sum(1 << (32 - (p + 1) * P_.Bgbit) for p in range(P_.l))
+    lo = np.int64(-(1 << (P_.Bgbit - 1)) * unit)
+    hi = np.int64(((1 << (P_.Bgbit - 1)) - 1) * unit)
+    wrap = lambda v: ((np.asarray(v, np.int64) + 2**31) % 2**32 - 2**31).astype(np.int32)
+    alt = np.where(np.arange(Np) % 2 == 0, lo, hi)
+    special = [np.full((2, Np), lo), np.full((2, Np), hi), np.stack([alt, alt[::-1]]),
+               np.full((2, Np), 2**31 - 1), np.full((2, Np), -2**31)]
+    return np.concatenate([rand_torus(rng, (n_rand, 2, Np)), wrap(np.stack(special))])
+
+
+def test_extern_product_fp64_exact_sweep(gpu_env):
+    """FP64 external products bit-exact with the exact int64 product over 48 products: rows spread
+    over the key, random and extreme-digit accumulators; the largest distance of the pre-rounding
+    value from the exact integer is bounded well below 1/2."""
+    _, dk, keys = gpu_env
+    rng = np.random.default_rng(1048)
+    acc = _extreme_accumulators(P, 43, rng)
+    rows = np.linspace(0, P.n - 1, acc.shape[0]).astype(np.int32)
+    got, raw = dk.extern_product(acc, rows, prec=0, want_raw=True)
+    worst = 0.0
+    for g in range(acc.shape[0]):
+        want, exact = O.extern_product_exact(P, acc[g], keys.bk[rows[g]], want_raw=True)
+        assert np.array_equal(got[g], want), (g, int(rows[g]))
+        worst = max(worst, float(np.abs(raw[g] - exact.astype(np.float64)).max()))
+    print(f"FP64 external product sweep ({acc.shape[0]} products): max |pre-rounding - exact| = {worst:.3e}")
+    assert worst < 0.125
