@@ -1157,11 +1157,11 @@ constexpr int KS_GATES = 256;
 #endif
 constexpr int KS_IBLK = TFHE_KS_IBLK;
 #ifndef TFHE_KS_WPE
-#define TFHE_KS_WPE 6     // wavefronts per SIMD the register allocation must allow
-#endif
+#define TFHE_KS_WPE 6     // wavefronts per SIMD the register allocation must allow (t = 8; at t = 10
+#endif                    // the kernel needs 104 VGPRs, so it asks for 4)
 
 template <int TKS, int KN, int NP, int C>
-__global__ void __launch_bounds__(KS_GATES) __attribute__((amdgpu_waves_per_eu(TFHE_KS_WPE))) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
+__global__ void __launch_bounds__(KS_GATES) __attribute__((amdgpu_waves_per_eu(TKS == 8 ? TFHE_KS_WPE : 4))) k_keyswitch(const int32_t* __restrict__ u1, const int32_t* __restrict__ u2,
                                                         int32_t add_b, const int32_t* __restrict__ ksk,
                                                         int32_t* __restrict__ out, int count, int n,
                                                         int32_t* const* __restrict__ out_tab, int out_seg) {
