@@ -134,6 +134,38 @@ def test_chained_gates_stay_correct(gpu_env, rng):
         assert np.array_equal(ks.decrypt(ct), bits), level
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+def test_million_chained_gates_no_decryption_failure(gpu_env, prec):
+    """Zero decryption failures in 10^6 gates on each path: 62 levels of 16384 NANDs (1,015,808
+    gates), each level's inputs the previous level's outputs under a random pairing, so every gate
+    sees bootstrapped and keyswitched noise.  Decryption on the device; the expected bits follow the
+    decrypted ones, so a failure is counted once."""
+    ks, dk, _ = gpu_env
+    batch, levels = 16384, 62
+    rng = np.random.default_rng(101 + prec)
+    bits = rng.integers(0, 2, batch).astype(np.int32)
+    cur = T().DeviceBuffer.from_array(ks.encrypt(bits))
+    other = T().DeviceBuffer(cur.nbytes)
+    out = T().DeviceBuffer(cur.nbytes)
+    fails = 0
+    try:
+        for level in range(levels):
+            perm = rng.permutation(batch)
+            pair = np.ascontiguousarray(cur.to_array((batch, P.n + 1))[perm])
+            assert T().lib().tfhe_gpu_memcpy_h2d(other.ptr, pair.ctypes.data_as(T()._vp), pair.nbytes) == 0
+            dk.gate_dev("NAND", out.ptr, cur.ptr, other.ptr, None, batch, prec=prec)
+            got = T().decrypt_dev(out, ks.lwe_key, batch)
+            want = 1 - (bits & bits[perm])
+            fails += int((got != want).sum())
+            bits = got
+            cur, out = out, cur
+    finally:
+        for b in (cur, other, out):
+            b.free()
+    assert batch * levels >= 10**6
+    assert fails == 0, (prec, fails)
+
+
 def test_upstream_api_boots_gates(gpu_env):
     """bootsNAND/bootsMUX/bootsCONSTANT through the upstream-compatible entry points."""
     ks, _, _ = gpu_env
