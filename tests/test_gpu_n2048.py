@@ -442,6 +442,47 @@ def test_mux_xor_8192_full_workload(env2048, prec):
             b.free()
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+def test_mux_xor_chained_million_gates_no_decryption_failure(env2048, prec):
+    """The N=2048 failure rate: zero decryption failures in 1,015,808 gates on each path, as 62
+    chained levels of the workload's shape (8192 MUX and 8192 XOR).  Each level draws its inputs
+    from the previous level's 16384 outputs under random pairings, so every gate sees bootstrapped
+    and keyswitched noise.  Decryption on the device; the expected bits follow the decrypted ones,
+    so a failure is counted once."""
+    ks, dk, _ = env2048
+    G, levels = 8192, 62
+    n1 = P.n + 1
+    rng = np.random.default_rng(2048 + prec)
+    bits = rng.integers(0, 2, 2 * G).astype(np.int32)
+    cur = T().DeviceBuffer.from_array(ks.encrypt(bits))
+    ins = [T().DeviceBuffer(2 * G * n1 * 4) for _ in range(3)]
+    out = T().DeviceBuffer(2 * G * n1 * 4)
+    half = G * n1 * 4
+    fails = 0
+    print(f"[n2048] chained MUX/XOR levels prec={prec}", flush=True)
+    try:
+        for level in range(levels):
+            host = cur.to_array((2 * G, n1))
+            perms = [rng.permutation(2 * G) for _ in range(3)]
+            for buf, pm in zip(ins, perms):
+                arr = np.ascontiguousarray(host[pm])
+                assert T().lib().tfhe_gpu_memcpy_h2d(buf.ptr, arr.ctypes.data_as(T()._vp), arr.nbytes) == 0
+            a, b, c = (buf.ptr.value for buf in ins)
+            dk.gate_dev("MUX", out.ptr.value, a, b, c, G, prec=prec)
+            dk.gate_dev("XOR", out.ptr.value + half, a + half, b + half, None, G, prec=prec)
+            got = T().decrypt_dev(out, ks.lwe_key, 2 * G)
+            x, y, z = (bits[pm] for pm in perms)
+            want = np.concatenate([np.where(x[:G] == 1, y[:G], z[:G]), x[G:] ^ y[G:]])
+            fails += int((got != want).sum())
+            bits = got
+            cur, out = out, cur
+    finally:
+        for buf in ins + [cur, out]:
+            buf.free()
+    assert 2 * G * levels >= 10**6
+    assert fails == 0, (prec, fails)
+
+
 def test_split_digits_fp16_n2048_bit_exact():
     """The N = 2048 split kernels' conversion of its 8-bit gadget digits to FP16 pairs (one byte
     permute, an or and one packed FP16 add) against the oracle's signed decomposition."""
