@@ -44,6 +44,21 @@ def test_library_exports_every_declared_function():
     assert set(header_functions()) <= exported
 
 
+def test_integration_guide_indexes_every_declared_function():
+    """The entry-point index of INTEGRATION.md names every function the header declares, and names
+    nothing the header lacks."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    index = text[text.index("## 5. Entry-point index"):]
+    declared = set(header_functions())
+    missing = [f for f in sorted(declared) if not re.search(r"`%s`" % re.escape(f), index)]
+    assert not missing, missing
+    named = set(re.findall(r"`([A-Za-z_][A-Za-z0-9_]*)`", index))
+    modules = {"tfhe_io"}   # upstream module names, not functions
+    unknown = sorted(n for n in named - modules
+                     if n not in declared and re.match(r"(boots|tfhe_|lwe|new_|delete_|export_|import_)", n))
+    assert not unknown, unknown
+
+
 def test_default_parameters_match_reference_constants():
     """kernel/src/include/constants.h:26-36 through the upstream constructor."""
     import tfhe_amd as T
